@@ -1,0 +1,177 @@
+"""Throughput of the fused D3Q27 BGK step on one MI355X.
+
+Workload: Taylor-Green vortex, 512^3 periodic, U0 = 0.05, Re = 1600, AA-pattern
+in-place storage -- the large TGV precision sweep.  A step is one time step
+over the whole 512^3 domain (one fused kernel launch).  The headline value is
+the FP64 (drop-in) throughput in GLUPS; the same run also steps FP32- and
+FP16-shifted storage from the same initial state and reports their GLUPS and
+their velocity L2 error against the FP64 run after the same steps.
+
+    python bench.py [--steps K] [--warmup W] [--n 512] [--storage aa|pull]
+
+Prints one JSON line.  Inputs are resident in HBM before the timed region; the
+timed region is K launches on the library's stream, bracketed by HIP events.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLUPS + % HBM roofline; FP64 vs FP16-shifted storage at stated TGV error"
+PEAK_GBS = 8000.0
+PI_R = 3.1415926
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tgv_tau(n, re, u0):
+    return u0 * n / re / (1.0 / 3.0) / 2 / PI_R
+
+
+def run_gpu(n, steps, warmup, storage, dtype, u0, re, block=0, cpt=0):
+    from imexlb_amd import LBM
+    g = LBM(n, n, n, tgv_tau(n, re, u0), u0=u0, case="tgv", engine="fused", dtype=dtype,
+            storage=storage, block=block, cells_per_thread=cpt)
+    g.Initialize()
+    g.step(warmup, moments_every=0)
+    g.sync()
+    ms = g.step_timed(steps, moments_every=0)
+    # one more step writes the moments used for the error report
+    g.step(1, moments_every=1)
+    info = g.info()
+    return g, ms, info
+
+
+def vel_l2(a, b):
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a[1:], b[1:]))
+    den = sum(float(np.sum(y ** 2)) for y in b[1:])
+    return (num / den) ** 0.5
+
+
+def traffic_from_profiles(workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(budget_s):
+    """The oracle (reference-structured CPU restatement, OpenMP) on a bounded
+    sample of the same workload: TGV 128^3, same U0 and Re, FP64."""
+    from oracle.oracle import TGV, Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    n = 128
+    o = Oracle(TGV, n, n, n, tau0=tgv_tau(n, 1600, 0.05), u0=0.05, threads=threads)
+    o.Initialize()
+    o.run(1)  # warm-up
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        o.run(1)
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or k >= 200:
+            break
+    glups = n ** 3 * k / el / 1e9
+    return {"value": glups, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle (reference phases, ghost 3, FP64, OpenMP) TGV {n}^3 Re 1600, "
+                      f"{k} steps in {el:.1f} s after 1 warm-up step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
+    ap.add_argument("--re", type=float, default=1600.0)
+    ap.add_argument("--u0", type=float, default=0.05)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lowp", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the multi-GPU halo over RCCL is a design "
+              "(see the sharding section of the design notes), not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import imexlb_amd  # noqa: F401  (fails loudly without libimexlb.so)
+    from imexlb_amd import copy_bandwidth
+
+    n, K, W = a.n, a.steps, a.warmup
+    cells = n ** 3
+    log(f"[bench] TGV {n}^3 {a.storage} FP64: warmup {W}, timed {K}")
+    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re)
+    glups64 = cells * K / (ms64 * 1e-3) / 1e9
+    per_launch_ms = ms64 / K
+    bytes_lup = info64["bytes_per_lup"]
+    achieved = bytes_lup * cells / (per_launch_ms * 1e-3) / 1e9
+    m64 = g64.moments()
+    log(f"[bench] FP64 {glups64:.2f} GLUPS, {achieved:.0f} GB/s")
+
+    extra = {}
+    if not a.no_lowp:
+        for dt in ("f32", "f16"):
+            log(f"[bench] TGV {n}^3 {a.storage} {dt}")
+            g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re)
+            gl = cells * K / (ms * 1e-3) / 1e9
+            m = g.moments()
+            err = vel_l2(m, m64)
+            bw = info["bytes_per_lup"] * cells / (ms / K * 1e-3) / 1e9
+            extra[f"{dt}_shifted"] = {"glups": gl, "ms_per_step": ms / K, "achieved_GBps": bw,
+                                      "frac": bw / PEAK_GBS, "vs_f64": gl / glups64,
+                                      "vel_l2_vs_f64": err, "after_steps": W + K + 1}
+            log(f"[bench] {dt} {gl:.2f} GLUPS ({gl / glups64:.2f}x FP64), L2 vs FP64 {err:.2e}")
+            del g, m
+    del g64, m64
+
+    gbps_copy, _ = copy_bandwidth(4 << 30, reps=10)
+    log(f"[bench] device copy {gbps_copy:.0f} GB/s")
+    workload = f"tgv{n}_{a.storage}_f64"
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(a.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": glups64,
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": per_launch_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Taylor-Green vortex D3Q27 BGK {n}^3 periodic, U0 {a.u0}, Re {a.re:g}, "
+                               f"{a.storage.upper()} storage, FP64 populations and collision",
+                   "cells": cells, "storage": a.storage, "re": a.re, "u0": a.u0},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_GBS, "traffic": traffic_from_profiles(workload),
+                     "bytes_per_lup": bytes_lup, "kernel": "k_fused (FP64, periodic, "
+                     + ("AA" if a.storage == "aa" else "pull") + ")",
+                     "measured_copy_GBps": gbps_copy},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,315 @@
+// Initialisation, population transfer, FP64 reductions and the copy kernel
+// used to calibrate bandwidth.  Compiled with -ffp-contract=off: the initial
+// state is the oracle's, bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "fused.h"
+#include "kernels.h"
+
+namespace imx {
+
+// feq in the reference's evaluation order (tgv/src/lbm.cpp:209-215).
+__device__ __forceinline__ double feq_f64(int i, double p, double u, double v, double w)
+{
+    const double edu = edu_of<double>(i, u, v, w);
+    const double udu = u * u + v * v + w * w;
+    const double eu2 = edu * edu;
+    return WVAL[WCLASS[i]] * p * 3.0 + WVAL[WCLASS[i]] * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
+}
+
+template <typename S>
+__device__ __forceinline__ void put(S *lat, uint64_t ps, int i, uint32_t c, double f, double shift)
+{
+    S *p = lat + (uint64_t)i * ps + c;
+    if constexpr (std::is_same<S, double>::value)
+        *p = f;
+    else if constexpr (std::is_same<S, float>::value)
+        *p = (float)(f - WVAL[WCLASS[i]] * shift);
+    else
+        *p = (_Float16)((float)(f - WVAL[WCLASS[i]] * shift) * H_SCALE[WCLASS[i]]);
+}
+
+template <typename S>
+__device__ __forceinline__ double get(const S *lat, uint64_t ps, int i, uint32_t c, double shift)
+{
+    const S *p = lat + (uint64_t)i * ps + c;
+    if constexpr (std::is_same<S, double>::value)
+        return *p;
+    else if constexpr (std::is_same<S, float>::value)
+        return (double)*p + WVAL[WCLASS[i]] * shift;
+    else
+        return (double)((float)*p * H_INV_SCALE[WCLASS[i]]) + WVAL[WCLASS[i]] * shift;
+}
+
+// Initial state = f0 = feq(p, u) of the analytic fields.  The reference's
+// first Collision relaxes f0 towards the feq of the same fields, which is f0
+// itself, so f0 is also the first post-collision state.
+template <typename S>
+__global__ void k_init(const InitParams P)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P.ncell) return;
+    const uint32_t t = P.divx.div(c);
+    const uint32_t x = c - t * P.nx;
+    const uint32_t z = P.divy.div(t);
+    const uint32_t y = t - z * P.ny;
+    double u, v, w, p;
+    if (P.kind == 0) {
+        // tgv/src/lbm.cpp:197-204 with per-axis sin/cos tables
+        const double *T = P.tab;
+        const double sx = T[x], cx = T[P.nx + x], c2x = T[2 * P.nx + x];
+        const double *Ty = T + 3 * P.nx;
+        const double sy = Ty[y], cy = Ty[P.ny + y], c2y = Ty[2 * P.ny + y];
+        const double *Tz = Ty + 3 * P.ny;
+        const double cz = Tz[P.nz + z], c2z = Tz[2 * P.nz + z];
+        u = P.u0 * sx * cy * cz;
+        v = -P.u0 * cx * sy * cz;
+        w = 0;
+        p = P.rho0 * P.cs2 + P.rho0 * P.u0 * P.u0 / 16.0 * (c2x + c2y) * (c2z + 2.0);
+    } else {
+        const bool solid = P.mask && P.mask[c] == 1;
+        u = solid ? 0.0 : P.u0;
+        v = 0.0;
+        w = 0.0;
+        p = P.p0;
+    }
+    S *lat = (S *)P.dst;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const double f = feq_f64(i, p, u, v, w);
+        put<S>(lat, P.pstride, P.layout == 1 ? BB[i] : i, c, f, P.shift);
+    }
+}
+
+hipError_t launch_init(int dtype, const InitParams &P, hipStream_t st)
+{
+    dim3 blk(256), grid((P.ncell + 255) / 256);
+    if (dtype == DT_F64) hipLaunchKernelGGL(k_init<double>, grid, blk, 0, st, P);
+    else if (dtype == DT_F32) hipLaunchKernelGGL(k_init<float>, grid, blk, 0, st, P);
+    else hipLaunchKernelGGL(k_init<_Float16>, grid, blk, 0, st, P);
+    return hipGetLastError();
+}
+
+// Lattice slot holding f*_i(x) for each storage layout.
+__device__ __forceinline__ void slot_of(const XferParams &P, int i, uint32_t c, int &slot, uint32_t &cell)
+{
+    const uint32_t nx = P.nx, ny = P.ny, nz = P.nz;
+    if (P.layout == 0) {
+        slot = i;
+        cell = c;
+        return;
+    }
+    if (P.layout == 1) {
+        slot = BB[i];
+        cell = c;
+        return;
+    }
+    // streamed (AA after an odd step): f*_i(x) sits at x + e_i in slot i, or
+    // at x in slot bb(i) when x + e_i is a cavity wall
+    const uint32_t t = P.divx.div(c);
+    int x = (int)(c - t * nx);
+    const uint32_t zz = P.divy.div(t);
+    int y = (int)(t - zz * ny), z = (int)zz;
+    int xd = x + EX[i], yd = y + EY[i], zd = z + EZ[i];
+    if (P.bc == BC_CAVITY) {
+        if (xd < 0 || xd >= (int)nx || yd < 0 || yd >= (int)ny || zd < 0 || zd >= (int)nz) {
+            slot = BB[i];
+            cell = c;
+            return;
+        }
+    } else {
+        xd = xd < 0 ? xd + (int)nx : (xd >= (int)nx ? xd - (int)nx : xd);
+        yd = yd < 0 ? yd + (int)ny : (yd >= (int)ny ? yd - (int)ny : yd);
+        zd = zd < 0 ? zd + (int)nz : (zd >= (int)nz ? zd - (int)nz : zd);
+    }
+    slot = i;
+    cell = ((uint32_t)zd * ny + (uint32_t)yd) * nx + (uint32_t)xd;
+}
+
+template <typename S>
+__global__ void k_get_f(const XferParams P)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P.ncell) return;
+    const S *lat = (const S *)P.lat;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int slot;
+        uint32_t cell;
+        slot_of(P, i, c, slot, cell);
+        double v = 0.0;
+        // slot is i or bb(i); both share the weight class of i
+        if (slot == i) v = get<S>(lat, P.pstride, i, cell, P.shift);
+        else v = get<S>(lat, P.pstride, BB[i], cell, P.shift);
+        P.buf[(uint64_t)i * P.ncell + c] = v;
+    }
+}
+
+template <typename S>
+__global__ void k_set_f(const XferParams P)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P.ncell) return;
+    S *lat = (S *)P.lat;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int slot;
+        uint32_t cell;
+        slot_of(P, i, c, slot, cell);
+        const double v = P.buf[(uint64_t)i * P.ncell + c];
+        if (slot == i) put<S>(lat, P.pstride, i, cell, v, P.shift);
+        else put<S>(lat, P.pstride, BB[i], cell, v, P.shift);
+    }
+}
+
+hipError_t launch_get_f(int dtype, const XferParams &P, hipStream_t st)
+{
+    dim3 blk(256), grid((P.ncell + 255) / 256);
+    if (dtype == DT_F64) hipLaunchKernelGGL(k_get_f<double>, grid, blk, 0, st, P);
+    else if (dtype == DT_F32) hipLaunchKernelGGL(k_get_f<float>, grid, blk, 0, st, P);
+    else hipLaunchKernelGGL(k_get_f<_Float16>, grid, blk, 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_f(int dtype, const XferParams &P, hipStream_t st)
+{
+    dim3 blk(256), grid((P.ncell + 255) / 256);
+    if (dtype == DT_F64) hipLaunchKernelGGL(k_set_f<double>, grid, blk, 0, st, P);
+    else if (dtype == DT_F32) hipLaunchKernelGGL(k_set_f<float>, grid, blk, 0, st, P);
+    else hipLaunchKernelGGL(k_set_f<_Float16>, grid, blk, 0, st, P);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ reductions
+// Two passes with a fixed assignment of cells to blocks and a fixed tree in
+// each block, so the sums are reproducible run to run.
+
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double (*sh)[256])
+{
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < N; ++k) sh[k][t] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) sh[k][t] += sh[k][t + s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = sh[k][0];
+}
+
+__global__ void __launch_bounds__(256) k_diag_pass1(const double *p, const double *u, const double *v,
+                                                    const double *w, uint32_t n, double *partials)
+{
+    __shared__ double sh[2][256];
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * per, hi = min(n, lo + per);
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+        acc[0] += u[c] * u[c] + v[c] * v[c] + w[c] * w[c];
+        acc[1] += 3.0 * p[c];
+    }
+    block_sum<2>(acc, sh);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = acc[0];
+        partials[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_sum_partials(const double *partials, int nparts, double *out, double scale0)
+{
+    __shared__ double sh[N][256];
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc[k] += partials[N * b + k];
+    block_sum<N>(acc, sh);
+    if (threadIdx.x == 0) {
+        out[0] = acc[0] * scale0;
+#pragma unroll
+        for (int k = 1; k < N; ++k) out[k] = acc[k];
+    }
+}
+
+hipError_t launch_diag(const double *p, const double *u, const double *v, const double *w, const uint8_t *,
+                       uint32_t n, double *partials, int nparts, double *out2, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_diag_pass1, dim3(nparts), dim3(256), 0, st, p, u, v, w, n, partials);
+    hipLaunchKernelGGL(k_sum_partials<2>, dim3(1), dim3(256), 0, st, partials, nparts, out2, 0.5);
+    return hipGetLastError();
+}
+
+// Momentum handed to the solid by halfway bounce-back, from the
+// post-collision populations of the channel (pull storage, natural layout):
+// F = sum over fluid x, directions i with x + e_i solid, of 2 f*_i(x) e_i.
+template <typename S>
+__global__ void __launch_bounds__(256) k_force_pass1(const FusedParams P, double *partials)
+{
+    __shared__ double sh[3][256];
+    const uint32_t n = P.ncell;
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * per, hi = min(n, lo + per);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const S *lat = (const S *)P.src;
+    for (uint32_t c = lo + threadIdx.x; c < hi; c += blockDim.x) {
+        if (P.mask[c] != 2) continue;
+        const uint32_t t = P.divx.div(c);
+        const int x = (int)(c - t * P.nx);
+        const uint32_t zz = P.divy.div(t);
+        const int y = (int)(t - zz * P.ny), z = (int)zz;
+#pragma unroll
+        for (int i = 1; i < Q; ++i) {
+            const int xd = x + EX[i];
+            if (xd < 0 || xd >= P.nx) continue;
+            int yd = y + EY[i], zd = z + EZ[i];
+            yd = yd < 0 ? yd + P.ny : (yd >= P.ny ? yd - P.ny : yd);
+            zd = zd < 0 ? zd + P.nz : (zd >= P.nz ? zd - P.nz : zd);
+            const uint32_t cd = ((uint32_t)zd * P.ny + (uint32_t)yd) * P.nx + (uint32_t)xd;
+            if (P.mask[cd] != 1) continue;
+            const double f = 2.0 * get<S>(lat, P.pstride, i, c, P.shift);
+            acc[0] += f * (double)EX[i];
+            acc[1] += f * (double)EY[i];
+            acc[2] += f * (double)EZ[i];
+        }
+    }
+    block_sum<3>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) partials[3 * blockIdx.x + k] = acc[k];
+}
+
+hipError_t launch_force(int dtype, const FusedParams &P, double *partials, int nparts, double *out3, hipStream_t st)
+{
+    if (dtype == DT_F64) hipLaunchKernelGGL(k_force_pass1<double>, dim3(nparts), dim3(256), 0, st, P, partials);
+    else if (dtype == DT_F32) hipLaunchKernelGGL(k_force_pass1<float>, dim3(nparts), dim3(256), 0, st, P, partials);
+    else hipLaunchKernelGGL(k_force_pass1<_Float16>, dim3(nparts), dim3(256), 0, st, P, partials);
+    hipLaunchKernelGGL(k_sum_partials<3>, dim3(1), dim3(256), 0, st, partials, nparts, out3, 1.0);
+    return hipGetLastError();
+}
+
+// Plain streaming copy, 16 B per lane per access: the reference point for the
+// hardware counters and the measured bandwidth ceiling.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(256) k_copy(const dvec2 *__restrict__ src, dvec2 *__restrict__ dst, size_t n2)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n2; i += stride) __builtin_nontemporal_store(src[i], dst + i);
+}
+
+hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st)
+{
+    const size_t n2 = n / 2;
+    const unsigned blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, (const dvec2 *)src, (dvec2 *)dst, n2);
+    return hipGetLastError();
+}
+
+}  // namespace imx

@@ -1,0 +1,700 @@
+// C-ABI of the MI355X D3Q27 BGK step: handle lifetime, the reference's
+// per-phase surface, the fused fast path and data access.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/imexlb.h"
+#include "kernels.h"
+#include "phased.h"
+
+using namespace imx;
+
+struct imexlb_handle {
+    imexlb_config cfg{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool own_stream = false;
+    // fused engine
+    void *lat[2] = {nullptr, nullptr};
+    int cur = 0;             // pull: lattice holding the current state
+    double *mom[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint8_t *mask = nullptr;
+    double *partials = nullptr;
+    double *result = nullptr;  // device scratch for reductions (4 doubles)
+    int nparts = 1024;
+    uint64_t pstride = 0;
+    uint32_t ncell = 0;
+    size_t elem = 8;
+    double shift = 0.0;
+    double corr[Q] = {};
+    double feq_in[Q] = {};
+    double *tgv_tab = nullptr;
+    int block = 256, cpt = 1, xcd = 1;
+    int64_t steps = 0;
+    int64_t mom_step = -1;   // step whose moments are in mom[]
+    bool initialized = false;
+    bool phase_collided = false;
+    int64_t device_bytes = 0;
+    // phased engine
+    PhasedState *ph = nullptr;
+    std::string err;
+};
+
+static thread_local std::string g_create_err;
+
+static int fail(imexlb_handle *h, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    else g_create_err = buf;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                           \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail((h), IMEXLB_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" {
+
+int imexlb_config_init(imexlb_config *c)
+{
+    if (!c) return IMEXLB_EINVAL;
+    std::memset(c, 0, sizeof *c);
+    c->case_id = IMEXLB_CASE_TGV;
+    c->mode = IMEXLB_MODE_FAITHFUL;
+    c->engine = IMEXLB_ENGINE_FUSED;
+    c->dtype = IMEXLB_F64;
+    c->storage = IMEXLB_PULL;
+    c->nx = c->ny = c->nz = 64;
+    c->tau0 = 0.05 * 64 / 100.0 / (1.0 / 3.0) / 2 / 3.1415926;
+    c->rho0 = 1.0;
+    c->u0 = 0.05;
+    c->lid_u = 0.1;
+    c->seed = 5374857ULL;
+    c->xcd_remap = -1;
+    return IMEXLB_OK;
+}
+
+const char *imexlb_strerror(int s)
+{
+    switch (s) {
+    case IMEXLB_OK: return "ok";
+    case IMEXLB_EINVAL: return "invalid argument";
+    case IMEXLB_EHIP: return "HIP runtime error";
+    case IMEXLB_ENOMEM: return "out of memory";
+    case IMEXLB_EUNSUPPORTED: return "not supported by this engine";
+    case IMEXLB_ESTATE: return "call out of order";
+    case IMEXLB_EIO: return "input file error";
+    case IMEXLB_ENODEV: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+const char *imexlb_last_error(const imexlb_handle *h)
+{
+    return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- helpers
+
+static int validate(const imexlb_config *c)
+{
+    if (c->case_id < 0 || c->case_id > 2) return fail(nullptr, IMEXLB_EINVAL, "case_id %d", c->case_id);
+    if (c->mode < 0 || c->mode > 1) return fail(nullptr, IMEXLB_EINVAL, "mode %d", c->mode);
+    if (c->engine < 0 || c->engine > 1) return fail(nullptr, IMEXLB_EINVAL, "engine %d", c->engine);
+    if (c->dtype < 0 || c->dtype > 2) return fail(nullptr, IMEXLB_EINVAL, "dtype %d", c->dtype);
+    if (c->storage < 0 || c->storage > 1) return fail(nullptr, IMEXLB_EINVAL, "storage %d", c->storage);
+    if (c->nx < 2 || c->ny < 2 || c->nz < 2)
+        return fail(nullptr, IMEXLB_EINVAL, "extents must be >= 2 (got %d %d %d)", c->nx, c->ny, c->nz);
+    const uint64_t n = (uint64_t)c->nx * c->ny * c->nz;
+    if (n >= (1ull << 31)) return fail(nullptr, IMEXLB_EINVAL, "subdomain of %llu cells exceeds 2^31", (unsigned long long)n);
+    if (!(c->tau0 > -0.5) || !std::isfinite(c->tau0))
+        return fail(nullptr, IMEXLB_EINVAL, "tau0 = %g gives no stable BGK rate", c->tau0);
+    if (!std::isfinite(c->rho0) || !std::isfinite(c->u0)) return fail(nullptr, IMEXLB_EINVAL, "non-finite rho0/u0");
+    if (c->block != 0 && (c->block < 64 || c->block > 512 || c->block % 64))
+        return fail(nullptr, IMEXLB_EINVAL, "block %d must be a multiple of 64 in [64, 512]", c->block);
+    if (c->cells_per_thread < 0 || c->cells_per_thread > 8)
+        return fail(nullptr, IMEXLB_EINVAL, "cells_per_thread %d", c->cells_per_thread);
+    if (c->engine == IMEXLB_ENGINE_FUSED) {
+        if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id != IMEXLB_CASE_TGV)
+            return fail(nullptr, IMEXLB_EUNSUPPORTED,
+                        "faithful cavity/sphere boundaries run on the phased engine");
+        if (c->case_id == IMEXLB_CASE_SPHERE && c->storage == IMEXLB_AA)
+            return fail(nullptr, IMEXLB_EUNSUPPORTED, "the open channel runs with pull storage");
+        if (c->case_id == IMEXLB_CASE_SPHERE) {
+            const double r = c->sphere_r;
+            if (!(r > 0) || c->sphere_cx - r < 2 || c->sphere_cx + r > c->nx - 3 || c->sphere_cy - r < 1 ||
+                c->sphere_cy + r > c->ny - 2 || c->sphere_cz - r < 1 || c->sphere_cz + r > c->nz - 2)
+                return fail(nullptr, IMEXLB_EINVAL, "sphere must lie inside the channel with a margin");
+        }
+    } else {
+        if (c->dtype != IMEXLB_F64 || c->storage != IMEXLB_PULL)
+            return fail(nullptr, IMEXLB_EUNSUPPORTED, "the phased engine is FP64 with its own layout");
+    }
+    return IMEXLB_OK;
+}
+
+static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst)
+{
+    const imexlb_config &c = h->cfg;
+    std::memset(&P, 0, sizeof P);
+    P.nx = c.nx;
+    P.ny = c.ny;
+    P.nz = c.nz;
+    P.ncell = h->ncell;
+    P.pstride = h->pstride;
+    P.src = h->lat[src];
+    P.dst = h->lat[dst];
+    P.mask = h->mask;
+    P.tau0 = c.tau0;
+    P.shift = h->shift;
+    std::memcpy(P.corr, h->corr, sizeof P.corr);
+    std::memcpy(P.feq_in, h->feq_in, sizeof P.feq_in);
+    const uint32_t tile = (uint32_t)h->block * (uint32_t)h->cpt;
+    P.nblocks = (h->ncell + tile - 1) / tile;
+    P.xcd_remap = h->xcd;
+    P.cpt = h->cpt;
+    P.divx.init(c.nx);
+    P.divy.init(c.ny);
+}
+
+namespace imx {
+hipError_t launch_fused_f64(int bc, int pattern, const FusedParams &P, int block, hipStream_t st);
+hipError_t launch_fused_f32(int bc, int pattern, const FusedParams &P, int block, hipStream_t st);
+hipError_t launch_fused_f16(int bc, int pattern, const FusedParams &P, int block, hipStream_t st);
+hipError_t launch_fused(int dtype, int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
+{
+    if (dtype == DT_F64) return launch_fused_f64(bc, pattern, P, block, st);
+    if (dtype == DT_F32) return launch_fused_f32(bc, pattern, P, block, st);
+    return launch_fused_f16(bc, pattern, P, block, st);
+}
+}  // namespace imx
+
+static int bc_of(const imexlb_config &c)
+{
+    return c.case_id == IMEXLB_CASE_TGV ? BC_PERIODIC : (c.case_id == IMEXLB_CASE_CAVITY ? BC_CAVITY : BC_CHANNEL);
+}
+
+// One fused step: step index k = h->steps + 1.
+static int fused_step(imexlb_handle *h, bool write_moments)
+{
+    const imexlb_config &c = h->cfg;
+    FusedParams P;
+    int pattern;
+    if (c.storage == IMEXLB_PULL) {
+        fill_fused_params(h, P, h->cur, 1 - h->cur);
+        pattern = PAT_PULL;
+    } else {
+        fill_fused_params(h, P, 0, 0);
+        // state after an even number of steps is the swapped layout, so the
+        // next (odd) step streams through neighbours
+        pattern = (h->steps % 2 == 0) ? PAT_AA_ODD : PAT_AA_EVEN;
+    }
+    if (write_moments) {
+        P.mom_p = h->mom[0];
+        P.mom_u = h->mom[1];
+        P.mom_v = h->mom[2];
+        P.mom_w = h->mom[3];
+    }
+    HIPCHK(h, launch_fused(c.dtype, bc_of(c), pattern, P, h->block, h->stream));
+    if (c.storage == IMEXLB_PULL) h->cur = 1 - h->cur;
+    h->steps++;
+    if (write_moments) h->mom_step = h->steps;
+    return IMEXLB_OK;
+}
+
+static int set_device(imexlb_handle *h)
+{
+    HIPCHK(h, hipSetDevice(h->device));
+    return IMEXLB_OK;
+}
+
+extern "C" {
+
+int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
+{
+    if (!cfg || !out) return fail(nullptr, IMEXLB_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = validate(cfg);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, IMEXLB_ENODEV, "no HIP device");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(nullptr, IMEXLB_EINVAL, "device %d of %d", cfg->device, ndev);
+
+    imexlb_handle *h = new (std::nothrow) imexlb_handle;
+    if (!h) return fail(nullptr, IMEXLB_ENOMEM, "handle");
+    h->cfg = *cfg;
+    h->device = cfg->device;
+    if ((rc = set_device(h))) {
+        g_create_err = h->err;
+        delete h;
+        return rc;
+    }
+    auto bail = [&](int code) {
+        g_create_err = h->err;
+        imexlb_destroy(h);
+        return code;
+    };
+    if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail(h, IMEXLB_EHIP, "hipStreamCreate"));
+    h->own_stream = true;
+    const imexlb_config &c = h->cfg;
+
+    if (c.engine == IMEXLB_ENGINE_PHASED) {
+        rc = phased_create(c, h->stream, &h->ph, h->err);
+        if (rc) return bail(rc);
+        h->device_bytes = phased_device_bytes(h->ph);
+        *out = h;
+        return IMEXLB_OK;
+    }
+
+    h->ncell = (uint32_t)((uint64_t)c.nx * c.ny * c.nz);
+    h->pstride = h->ncell;
+    h->elem = c.dtype == IMEXLB_F64 ? 8 : (c.dtype == IMEXLB_F32 ? 4 : 2);
+    h->block = c.block ? c.block : 256;
+    h->cpt = c.cells_per_thread ? c.cells_per_thread : 1;
+    h->xcd = c.xcd_remap < 0 ? 1 : c.xcd_remap;
+    // stored shift S: f = g + w S for reduced precision; FP64 stores f itself
+    if (c.dtype != IMEXLB_F64) h->shift = c.rho0;
+    const size_t lat_bytes = (size_t)Q * h->pstride * h->elem;
+    const int nlat = c.storage == IMEXLB_PULL ? 2 : 1;
+    for (int k = 0; k < nlat; ++k) {
+        if (hipMalloc(&h->lat[k], lat_bytes) != hipSuccess)
+            return bail(fail(h, IMEXLB_ENOMEM, "lattice of %zu bytes", lat_bytes));
+        h->device_bytes += lat_bytes;
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (hipMalloc(&h->mom[k], (size_t)h->ncell * 8) != hipSuccess)
+            return bail(fail(h, IMEXLB_ENOMEM, "moments"));
+        h->device_bytes += (int64_t)h->ncell * 8;
+        if (hipMemsetAsync(h->mom[k], 0, (size_t)h->ncell * 8, h->stream) != hipSuccess)
+            return bail(fail(h, IMEXLB_EHIP, "memset"));
+    }
+    if (hipMalloc(&h->partials, sizeof(double) * 4 * h->nparts) != hipSuccess ||
+        hipMalloc(&h->result, sizeof(double) * 4) != hipSuccess)
+        return bail(fail(h, IMEXLB_ENOMEM, "reduction scratch"));
+
+    // boundary constants in the oracle's operation order
+    const double cs2 = 1.0 / 3.0;
+    if (c.case_id == IMEXLB_CASE_CAVITY) {
+        for (int i = 0; i < Q; ++i) h->corr[i] = 6.0 * w(i) * c.rho0 * ((double)EX[i] * c.lid_u);
+    }
+    if (c.case_id == IMEXLB_CASE_SPHERE) {
+        const double pin = c.rho0 * cs2;
+        for (int i = 0; i < Q; ++i) {
+            const double u = c.u0, v = 0.0, ww = 0.0;
+            const double edu = (double)EX[i] * u + (double)EY[i] * v + (double)EZ[i] * ww;
+            const double udu = u * u + v * v + ww * ww;
+            const double eu2 = edu * edu;
+            const double feq = w(i) * pin * 3.0 + w(i) * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
+            h->feq_in[i] = c.dtype == IMEXLB_F64 ? feq : feq - w(i) * h->shift;
+        }
+        // mask: 1 solid, 2 fluid with a solid neighbour (y, z periodic)
+        std::vector<uint8_t> m((size_t)h->ncell, 0);
+        const double r2 = c.sphere_r * c.sphere_r;
+        auto solid = [&](int x, int y, int z) {
+            const double dx = (double)x - c.sphere_cx, dy = (double)y - c.sphere_cy, dz = (double)z - c.sphere_cz;
+            return dx * dx + dy * dy + dz * dz <= r2;
+        };
+        const int x0 = std::max(0, (int)std::floor(c.sphere_cx - c.sphere_r) - 2);
+        const int x1 = std::min(c.nx - 1, (int)std::ceil(c.sphere_cx + c.sphere_r) + 2);
+        const int y0 = std::max(0, (int)std::floor(c.sphere_cy - c.sphere_r) - 2);
+        const int y1 = std::min(c.ny - 1, (int)std::ceil(c.sphere_cy + c.sphere_r) + 2);
+        const int z0 = std::max(0, (int)std::floor(c.sphere_cz - c.sphere_r) - 2);
+        const int z1 = std::min(c.nz - 1, (int)std::ceil(c.sphere_cz + c.sphere_r) + 2);
+        auto at = [&](int x, int y, int z) -> uint8_t & { return m[((size_t)z * c.ny + y) * c.nx + x]; };
+        for (int z = z0; z <= z1; ++z)
+            for (int y = y0; y <= y1; ++y)
+                for (int x = x0; x <= x1; ++x)
+                    if (solid(x, y, z)) at(x, y, z) = 1;
+        for (int z = z0; z <= z1; ++z)
+            for (int y = y0; y <= y1; ++y)
+                for (int x = x0; x <= x1; ++x) {
+                    if (at(x, y, z) == 1) continue;
+                    for (int i = 1; i < Q; ++i) {
+                        const int xx = x + EX[i], yy = y + EY[i], zz = z + EZ[i];
+                        if (xx < 0 || xx >= c.nx || yy < 0 || yy >= c.ny || zz < 0 || zz >= c.nz) continue;
+                        if (at(xx, yy, zz) == 1) {
+                            at(x, y, z) = 2;
+                            break;
+                        }
+                    }
+                }
+        if (hipMalloc(&h->mask, h->ncell) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "mask"));
+        h->device_bytes += h->ncell;
+        if (hipMemcpy(h->mask, m.data(), h->ncell, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(h, IMEXLB_EHIP, "mask upload"));
+    }
+    if (c.case_id == IMEXLB_CASE_TGV) {
+        // per-axis tables of the TGV initial field (tgv/src/lbm.cpp:197-204)
+        std::vector<double> tab(3 * ((size_t)c.nx + c.ny + c.nz));
+        size_t o = 0;
+        const int ext[3] = {c.nx, c.ny, c.nz};
+        for (int a = 0; a < 3; ++a) {
+            const int n = ext[a];
+            for (int k = 0; k < n; ++k) tab[o + k] = std::sin((double)k / (double)n * 2.0 * PI_REF);
+            for (int k = 0; k < n; ++k) tab[o + n + k] = std::cos((double)k / (double)n * 2.0 * PI_REF);
+            for (int k = 0; k < n; ++k) tab[o + 2 * n + k] = std::cos((double)k / (double)n * 2.0 * PI_REF * 2.0);
+            o += 3 * (size_t)n;
+        }
+        if (hipMalloc(&h->tgv_tab, tab.size() * 8) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "tables"));
+        if (hipMemcpy(h->tgv_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(h, IMEXLB_EHIP, "table upload"));
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(h, IMEXLB_EHIP, "sync"));
+    *out = h;
+    return IMEXLB_OK;
+}
+
+int imexlb_destroy(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_OK;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto &p : h->lat)
+        if (p) hipFree(p);
+    for (auto &p : h->mom)
+        if (p) hipFree(p);
+    if (h->mask) hipFree(h->mask);
+    if (h->partials) hipFree(h->partials);
+    if (h->result) hipFree(h->result);
+    if (h->tgv_tab) hipFree(h->tgv_tab);
+    if (h->ph) phased_destroy(h->ph);
+    if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
+    delete h;
+    return IMEXLB_OK;
+}
+
+int imexlb_initialize(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    int rc = set_device(h);
+    if (rc) return rc;
+    const imexlb_config &c = h->cfg;
+    if (h->ph) {
+        rc = phased_initialize(h->ph, h->err);
+        if (rc) return rc;
+        h->initialized = true;
+        h->steps = 0;
+        return IMEXLB_OK;
+    }
+    InitParams P;
+    std::memset(&P, 0, sizeof P);
+    P.nx = c.nx;
+    P.ny = c.ny;
+    P.nz = c.nz;
+    P.ncell = h->ncell;
+    P.pstride = h->pstride;
+    P.dst = h->lat[0];
+    P.layout = c.storage == IMEXLB_AA ? 1 : 0;
+    P.kind = c.case_id == IMEXLB_CASE_TGV ? 0 : 1;
+    P.tab = h->tgv_tab;
+    P.u0 = c.case_id == IMEXLB_CASE_SPHERE ? c.u0 : (c.case_id == IMEXLB_CASE_CAVITY ? 0.0 : c.u0);
+    P.rho0 = c.rho0;
+    P.cs2 = 1.0 / 3.0;
+    P.p0 = c.rho0 * P.cs2;
+    P.shift = h->shift;
+    P.mask = h->mask;
+    P.divx.init(c.nx);
+    P.divy.init(c.ny);
+    HIPCHK(h, launch_init(c.dtype, P, h->stream));
+    h->cur = 0;
+    if (c.storage == IMEXLB_PULL && c.case_id == IMEXLB_CASE_SPHERE) {
+        // solid cells are never written by the step; give both lattices the
+        // same initial contents
+        HIPCHK(h, hipMemcpyAsync(h->lat[1], h->lat[0], (size_t)Q * h->pstride * h->elem, hipMemcpyDeviceToDevice,
+                                 h->stream));
+    }
+    h->steps = 0;
+    h->mom_step = -1;
+    h->initialized = true;
+    h->phase_collided = false;
+    return IMEXLB_OK;
+}
+
+int imexlb_setup_subdomain(imexlb_handle *h) { return h ? IMEXLB_OK : IMEXLB_EINVAL; }
+
+int imexlb_collide(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (!h->initialized) return fail(h, IMEXLB_ESTATE, "Collision before Initialize");
+    if (h->ph) return phased_collide(h->ph, h->err);
+    h->phase_collided = true;  // fused: the collision happens inside Streaming
+    return IMEXLB_OK;
+}
+
+int imexlb_pack(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (h->ph) return phased_pack(h->ph, h->err);
+    return IMEXLB_OK;  // fused: periodic images are read through the index wrap
+}
+
+int imexlb_exchange(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (h->ph) return phased_exchange(h->ph, h->err);
+    return IMEXLB_OK;
+}
+
+int imexlb_unpack(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (h->ph) return phased_unpack(h->ph, h->err);
+    return IMEXLB_OK;
+}
+
+int imexlb_stream(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (!h->initialized) return fail(h, IMEXLB_ESTATE, "Streaming before Initialize");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (h->ph) return phased_stream(h->ph, h->err);
+    if (!h->phase_collided) return fail(h, IMEXLB_ESTATE, "Streaming without a preceding Collision");
+    h->phase_collided = false;
+    return fused_step(h, true);
+}
+
+int imexlb_update(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (h->ph) {
+        int rc = phased_update(h->ph, h->err);
+        if (!rc) h->steps++;
+        return rc;
+    }
+    return IMEXLB_OK;  // fused: moments were written by Streaming
+}
+
+int imexlb_step(imexlb_handle *h, int nsteps, int moments_every)
+{
+    if (!h || nsteps < 0 || moments_every < 0) return IMEXLB_EINVAL;
+    if (!h->initialized) return fail(h, IMEXLB_ESTATE, "step before Initialize");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (h->ph) {
+        for (int k = 0; k < nsteps; ++k) {
+            if ((rc = phased_full_step(h->ph, h->err))) return rc;
+            h->steps++;
+        }
+        return IMEXLB_OK;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        const int64_t idx = h->steps + 1;
+        const bool wm = moments_every > 0 && (idx % moments_every == 0 || k == nsteps - 1);
+        if ((rc = fused_step(h, wm))) return rc;
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_step_timed(imexlb_handle *h, int nsteps, int moments_every, float *ms)
+{
+    if (!h || !ms) return IMEXLB_EINVAL;
+    int rc = set_device(h);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+    rc = imexlb_step(h, nsteps, moments_every);
+    if (rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+    }
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    HIPCHK(h, hipEventSynchronize(e1));
+    HIPCHK(h, hipEventElapsedTime(ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return IMEXLB_OK;
+}
+
+int imexlb_sync(imexlb_handle *h)
+{
+    if (!h) return IMEXLB_EINVAL;
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return IMEXLB_OK;
+}
+
+int imexlb_get_moments(imexlb_handle *h, double *p, double *u, double *v, double *w)
+{
+    if (!h) return IMEXLB_EINVAL;
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (h->ph) return phased_get_moments(h->ph, p, u, v, w, h->err);
+    if (h->mom_step != h->steps)
+        return fail(h, IMEXLB_ESTATE, "no moments for step %lld (last written at %lld)", (long long)h->steps,
+                    (long long)h->mom_step);
+    double *out[4] = {p, u, v, w};
+    for (int k = 0; k < 4; ++k)
+        if (out[k]) HIPCHK(h, hipMemcpyAsync(out[k], h->mom[k], (size_t)h->ncell * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return IMEXLB_OK;
+}
+
+static int xfer(imexlb_handle *h, double *host, bool get)
+{
+    const imexlb_config &c = h->cfg;
+    XferParams P;
+    std::memset(&P, 0, sizeof P);
+    P.nx = c.nx;
+    P.ny = c.ny;
+    P.nz = c.nz;
+    P.ncell = h->ncell;
+    P.pstride = h->pstride;
+    P.shift = h->shift;
+    P.bc = bc_of(c);
+    std::memcpy(P.corr, h->corr, sizeof P.corr);
+    if (c.storage == IMEXLB_PULL) {
+        P.lat = h->lat[h->cur];
+        P.layout = 0;
+    } else {
+        P.lat = h->lat[0];
+        P.layout = (h->steps % 2 == 0) ? 1 : 2;
+    }
+    P.divx.init(c.nx);
+    P.divy.init(c.ny);
+    const size_t bytes = (size_t)Q * h->ncell * 8;
+    double *d = nullptr;
+    HIPCHK(h, hipMalloc(&d, bytes));
+    P.buf = d;
+    hipError_t e = hipSuccess;
+    if (get) {
+        e = launch_get_f(c.dtype, P, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, h->stream);
+    } else {
+        e = hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = launch_set_f(c.dtype, P, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(h, IMEXLB_EHIP, "population transfer: %s", hipGetErrorString(e));
+    return IMEXLB_OK;
+}
+
+int imexlb_get_f(imexlb_handle *h, double *f)
+{
+    if (!h || !f) return IMEXLB_EINVAL;
+    if (!h->initialized) return fail(h, IMEXLB_ESTATE, "get_f before Initialize");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (h->ph) return phased_get_f(h->ph, f, h->err);
+    return xfer(h, f, true);
+}
+
+int imexlb_set_f(imexlb_handle *h, const double *f)
+{
+    if (!h || !f) return IMEXLB_EINVAL;
+    if (!h->initialized) return fail(h, IMEXLB_ESTATE, "set_f before Initialize");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (h->ph) return phased_set_f(h->ph, f, h->err);
+    h->mom_step = -1;
+    return xfer(h, const_cast<double *>(f), false);
+}
+
+int imexlb_diagnostics(imexlb_handle *h, double *ke, double *mass)
+{
+    if (!h || !ke || !mass) return IMEXLB_EINVAL;
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (h->ph) return phased_diagnostics(h->ph, ke, mass, h->err);
+    if (h->mom_step != h->steps) return fail(h, IMEXLB_ESTATE, "diagnostics need the moments of the current step");
+    HIPCHK(h, launch_diag(h->mom[0], h->mom[1], h->mom[2], h->mom[3], h->mask, h->ncell, h->partials, h->nparts,
+                          h->result, h->stream));
+    double out[2];
+    HIPCHK(h, hipMemcpyAsync(out, h->result, sizeof out, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *ke = out[0];
+    *mass = out[1];
+    return IMEXLB_OK;
+}
+
+int imexlb_sphere_force(imexlb_handle *h, double *force3)
+{
+    if (!h || !force3) return IMEXLB_EINVAL;
+    if (h->ph || h->cfg.case_id != IMEXLB_CASE_SPHERE)
+        return fail(h, IMEXLB_EUNSUPPORTED, "sphere force is defined for the config-mode channel");
+    int rc = set_device(h);
+    if (rc) return rc;
+    FusedParams P;
+    fill_fused_params(h, P, h->cur, h->cur);
+    HIPCHK(h, launch_force(h->cfg.dtype, P, h->partials, h->nparts, h->result, h->stream));
+    HIPCHK(h, hipMemcpyAsync(force3, h->result, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return IMEXLB_OK;
+}
+
+int imexlb_get_info(imexlb_handle *h, imexlb_info *info)
+{
+    if (!h || !info) return IMEXLB_EINVAL;
+    std::memset(info, 0, sizeof *info);
+    const imexlb_config &c = h->cfg;
+    info->cells = (int64_t)c.nx * c.ny * c.nz;
+    info->steps = h->steps;
+    const double e = c.dtype == IMEXLB_F64 ? 8.0 : (c.dtype == IMEXLB_F32 ? 4.0 : 2.0);
+    info->bytes_per_lup = 2.0 * Q * (h->ph ? 8.0 : e) + (c.case_id == IMEXLB_CASE_SPHERE && !h->ph ? 1.0 : 0.0);
+    info->device_bytes = h->device_bytes;
+    info->block = h->block;
+    info->cells_per_thread = h->cpt;
+    info->xcd_remap = h->xcd;
+    return IMEXLB_OK;
+}
+
+int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch)
+{
+    if (bytes < 1024 || reps < 1 || !gbps || !ms_per_launch) return IMEXLB_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return IMEXLB_ENODEV;
+    const size_t n = bytes / 8 / 2 * 2;
+    double *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 8) != hipSuccess) return IMEXLB_ENOMEM;
+    if (hipMalloc(&b, n * 8) != hipSuccess) {
+        hipFree(a);
+        return IMEXLB_ENOMEM;
+    }
+    hipStream_t st;
+    hipEvent_t e0, e1;
+    int rc = IMEXLB_OK;
+    if (hipStreamCreate(&st) != hipSuccess) rc = IMEXLB_EHIP;
+    if (!rc && (hipMemsetAsync(a, 0, n * 8, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        rc = IMEXLB_EHIP;
+    if (!rc) {
+        launch_copy(a, b, n, st);  // warm-up
+        hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) launch_copy(a, b, n, st);
+        hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = IMEXLB_EHIP;
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        *ms_per_launch = ms / reps;
+        *gbps = 2.0 * (double)n * 8.0 / (*ms_per_launch * 1e-3) / 1e9;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipStreamDestroy(st);
+    }
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
+}  // extern "C"

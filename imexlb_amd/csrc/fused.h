@@ -1,0 +1,338 @@
+// The fused D3Q27 BGK step: stream (pull or AA), moments, collide, in one
+// pass over HBM.  Included by fused_f64.hip (compiled with -ffp-contract=off,
+// bit-for-bit the oracle's FP64 arithmetic) and fused_lp.hip (FP32/FP16
+// storage, FP32 collision).
+//
+// What one launch computes, per fluid cell x, equals one iteration of the
+// reference's driver loop (tgv/main.cpp:70-119): the post-collision
+// populations of the previous step are streamed (Streaming, tgv/src/lbm.cpp:
+// 239-256), their moments taken (Update, tgv/src/lbm.cpp:288-333) and relaxed
+// towards equilibrium (Collision, tgv/src/lbm.cpp:222-237) for the next step.
+// The one-rank halo (tgv/src/mpi_view_transfer.cpp:117-383) is the periodic
+// index wrap below.
+//
+// Layout: structure of arrays, population i at lattice + i * pstride, cells in
+// x-fastest order, 64 consecutive x cells per wavefront.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <type_traits>
+
+#include "kernels.h"
+
+namespace imx {
+
+template <typename S> struct RealOf { using type = float; };
+template <> struct RealOf<double> { using type = double; };
+
+template <typename S>
+__device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_t pstride, int i, uint32_t c)
+{
+    const S *p = lat + (uint64_t)i * pstride + c;
+    if constexpr (std::is_same<S, _Float16>::value)
+        return (float)(*p) * H_INV_SCALE[WCLASS[i]];
+    else
+        return *p;
+}
+
+template <typename S>
+__device__ __forceinline__ void st_pop(S *lat, uint64_t pstride, int i, uint32_t c, typename RealOf<S>::type v)
+{
+    S *p = lat + (uint64_t)i * pstride + c;
+    if constexpr (std::is_same<S, _Float16>::value)
+        __builtin_nontemporal_store((_Float16)(v * H_SCALE[WCLASS[i]]), p);
+    else
+        __builtin_nontemporal_store(v, p);
+}
+
+// XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
+// robin; this gives XCD r one contiguous run of tiles so that the partially
+// used cache lines at tile edges stay in one L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb)
+{
+    const uint32_t xcd = b & 7u, idx = b >> 3, per = nb >> 3, rem = nb & 7u;
+    return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// e_i . u with the zero components left out, summed x, y, z in that order --
+// the value of ((e_x u + e_y v) + e_z w) for e in {-1, 0, 1}^3.
+template <typename R>
+__device__ __forceinline__ R edu_of(int i, R u, R v, R w)
+{
+    R acc = R(0);
+    bool has = false;
+    if (EX[i] != 0) {
+        acc = EX[i] > 0 ? u : -u;
+        has = true;
+    }
+    if (EY[i] != 0) {
+        R t = EY[i] > 0 ? v : -v;
+        acc = has ? acc + t : t;
+        has = true;
+    }
+    if (EZ[i] != 0) {
+        R t = EZ[i] > 0 ? w : -w;
+        acc = has ? acc + t : t;
+    }
+    return acc;
+}
+
+// Moments in direction order 0..26 (lid_driven_cavity/src/lbm.cpp:329-347):
+// p = sum f/3, u = sum f e.  FP64 storage: exactly the oracle's sums.
+__device__ __forceinline__ void moments_f64(const double (&f)[Q], double &pm, double &um, double &vm, double &wm)
+{
+    pm = 0.0;
+    um = 0.0;
+    vm = 0.0;
+    wm = 0.0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        pm = pm + f[i] / 3.0;
+        if (EX[i] > 0) um = um + f[i];
+        if (EX[i] < 0) um = um - f[i];
+        if (EY[i] > 0) vm = vm + f[i];
+        if (EY[i] < 0) vm = vm - f[i];
+        if (EZ[i] > 0) wm = wm + f[i];
+        if (EZ[i] < 0) wm = wm - f[i];
+    }
+}
+
+// BGK relaxation in the reference's operation order (tgv/src/lbm.cpp:228-234):
+// feq = t p 3 + t (3 e.u + 4.5 (e.u)^2 - 1.5 u.u); f -= (f - feq) / (tau0 + 0.5).
+__device__ __forceinline__ void collide_f64(double (&f)[Q], double pm, double u, double v, double w, double tau0)
+{
+    const double den = tau0 + 0.5;
+    const double udu = u * u + v * v + w * w;
+    const double u15 = 1.5 * udu;
+    double tp3[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) tp3[k] = WVAL[k] * pm * 3.0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const double edu = edu_of<double>(i, u, v, w);
+        const double eu2 = edu * edu;
+        const double feq = tp3[WCLASS[i]] + WVAL[WCLASS[i]] * (3.0 * edu + 4.5 * eu2 - u15);
+        f[i] = f[i] - (f[i] - feq) / den;
+    }
+}
+
+// Reduced-precision storage holds g = f - w S.  The incompressible BGK update
+// is linear in p, so g obeys the same update with p' = p - S/3; the density
+// sum is kept in FP64, the rest in FP32.
+__device__ __forceinline__ void moments_lp(const float (&g)[Q], double &pd, float &u, float &v, float &w)
+{
+    pd = 0.0;
+    u = 0.f;
+    v = 0.f;
+    w = 0.f;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        pd += (double)g[i];
+        if (EX[i] > 0) u += g[i];
+        if (EX[i] < 0) u -= g[i];
+        if (EY[i] > 0) v += g[i];
+        if (EY[i] < 0) v -= g[i];
+        if (EZ[i] > 0) w += g[i];
+        if (EZ[i] < 0) w -= g[i];
+    }
+}
+
+__device__ __forceinline__ void collide_lp(float (&g)[Q], double pd, float u, float v, float w, float omega)
+{
+    const float p3 = (float)pd;  // 3 p' = sum g
+    const float u15 = 1.5f * (u * u + v * v + w * w);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const float edu = edu_of<float>(i, u, v, w);
+        const float feq = (float)WVAL[WCLASS[i]] * (p3 + 3.f * edu + 4.5f * edu * edu - u15);
+        g[i] = g[i] + omega * (feq - g[i]);
+    }
+}
+
+template <typename S, int BC, int PAT>
+__device__ __forceinline__ void fused_cell(const FusedParams &P, uint32_t c)
+{
+    using R = typename RealOf<S>::type;
+    constexpr bool F64 = std::is_same<S, double>::value;
+    const uint32_t nx = P.nx, ny = P.ny, nz = P.nz;
+    const uint32_t t = P.divx.div(c);
+    const uint32_t x = c - t * nx;
+    const uint32_t z = P.divy.div(t);
+    const uint32_t y = t - z * ny;
+
+    uint8_t m = 0;
+    if constexpr (BC == BC_CHANNEL) {
+        m = P.mask[c];
+        if (m == 1) return;  // solid: nothing streams from or to it
+    }
+
+    // neighbour coordinates at x - 1, x, x + 1 (index 0, 1, 2) per axis:
+    // periodic axes wrap, walls clamp (the clamped value is never used as a
+    // source), the channel outlet clamps (zero-gradient ghost).
+    uint32_t xv[3], yv[3], zv[3];
+    if constexpr (BC == BC_PERIODIC) {
+        xv[0] = x == 0 ? nx - 1 : x - 1;
+        xv[2] = x == nx - 1 ? 0 : x + 1;
+    } else {
+        xv[0] = x == 0 ? 0 : x - 1;
+        xv[2] = x == nx - 1 ? nx - 1 : x + 1;
+    }
+    if constexpr (BC == BC_CAVITY) {
+        yv[0] = y == 0 ? 0 : y - 1;
+        yv[2] = y == ny - 1 ? ny - 1 : y + 1;
+        zv[0] = z == 0 ? 0 : z - 1;
+        zv[2] = z == nz - 1 ? nz - 1 : z + 1;
+    } else {
+        yv[0] = y == 0 ? ny - 1 : y - 1;
+        yv[2] = y == ny - 1 ? 0 : y + 1;
+        zv[0] = z == 0 ? nz - 1 : z - 1;
+        zv[2] = z == nz - 1 ? 0 : z + 1;
+    }
+    xv[1] = x;
+    yv[1] = y;
+    zv[1] = z;
+    uint32_t rowb[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) rowb[a][b] = (zv[a] * ny + yv[b]) * nx;
+
+    // wall flags: is the cell at x - e_i outside the cavity?
+    const bool x0 = x == 0, x1 = x == nx - 1, y0 = y == 0, y1 = y == ny - 1, z0 = z == 0, z1 = z == nz - 1;
+    auto src_out = [&](int i) -> bool {
+        return (EX[i] > 0 && x0) || (EX[i] < 0 && x1) || (EY[i] > 0 && y0) || (EY[i] < 0 && y1) ||
+               (EZ[i] > 0 && z0) || (EZ[i] < 0 && z1);
+    };
+    // offset of the cell x - e_i
+    auto src_off = [&](int i) -> uint32_t { return rowb[1 - EZ[i]][1 - EY[i]] + xv[1 - EX[i]]; };
+
+    const S *src = (const S *)P.src;
+    S *dst = (S *)P.dst;
+    const uint64_t ps = P.pstride;
+
+    R f[Q];
+    if constexpr (PAT == PAT_AA_EVEN) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            f[i] = ld_pop<S>(src, ps, i, c);
+            if constexpr (BC == BC_CAVITY) {
+                // the odd step stored the bounced population raw; the lid
+                // term is added on arrival, as in the pull form
+                if (EZ[i] < 0 && z1) {
+                    if constexpr (F64) f[i] = f[i] + P.corr[i];
+                    else f[i] = f[i] + (float)P.corr[i];
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if constexpr (BC == BC_PERIODIC) {
+                f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(i));
+            } else if constexpr (BC == BC_CAVITY) {
+                if (!src_out(i)) {
+                    f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(i));
+                } else {
+                    // halfway bounce-back: f_i(x) = f*_bb(i)(x) (+ lid term)
+                    R v = ld_pop<S>(src, ps, PAT == PAT_PULL ? BB[i] : i, c);
+                    if (EZ[i] < 0 && z1) {
+                        if constexpr (F64) v = v + P.corr[i];
+                        else v = v + (float)P.corr[i];
+                    }
+                    f[i] = v;
+                }
+            } else {  // channel, pull only
+                if (EX[i] > 0 && x0) {
+                    f[i] = F64 ? (R)P.feq_in[i] : (R)(float)P.feq_in[i];
+                } else {
+                    const uint32_t so = src_off(i);
+                    bool solid = false;
+                    if (m == 2 && i != 0) solid = P.mask[so] == 1;
+                    if (!solid) f[i] = ld_pop<S>(src, ps, i, so);
+                    else f[i] = ld_pop<S>(src, ps, BB[i], c);
+                }
+            }
+        }
+    }
+
+    if constexpr (F64) {
+        double pm, um, vm, wm;
+        moments_f64(f, pm, um, vm, wm);
+        if (P.mom_p) {
+            P.mom_p[c] = pm;
+            P.mom_u[c] = um;
+            P.mom_v[c] = vm;
+            P.mom_w[c] = wm;
+        }
+        collide_f64(f, pm, um, vm, wm, P.tau0);
+    } else {
+        double pd;
+        float u, v, w;
+        moments_lp(f, pd, u, v, w);
+        if (P.mom_p) {
+            P.mom_p[c] = (pd + P.shift) / 3.0;
+            P.mom_u[c] = u;
+            P.mom_v[c] = v;
+            P.mom_w[c] = w;
+        }
+        const float omega = (float)(1.0 / (P.tau0 + 0.5));
+        collide_lp(f, pd, u, v, w, omega);
+    }
+
+    if constexpr (PAT == PAT_PULL) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) st_pop<S>(dst, ps, i, c, f[i]);
+    } else if constexpr (PAT == PAT_AA_EVEN) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) st_pop<S>(dst, ps, BB[i], c, f[i]);
+    } else {
+        // to the natural slot of x + e_i, or bounced back into slot bb(i) at x
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if constexpr (BC == BC_CAVITY) {
+                if (src_out(BB[i])) {
+                    st_pop<S>(dst, ps, BB[i], c, f[i]);
+                    continue;
+                }
+            }
+            st_pop<S>(dst, ps, i, src_off(BB[i]), f[i]);
+        }
+    }
+}
+
+template <typename S, int BC, int PAT>
+__global__ void __launch_bounds__(512) k_fused(const FusedParams P)
+{
+    uint32_t b = blockIdx.x;
+    if (P.xcd_remap) b = xcd_tile(b, P.nblocks);
+    const uint32_t tile = blockDim.x * (uint32_t)P.cpt;
+    const uint32_t base = b * tile + threadIdx.x;
+    for (int r = 0; r < P.cpt; ++r) {
+        const uint32_t c = base + (uint32_t)r * blockDim.x;
+        if (c >= P.ncell) return;
+        fused_cell<S, BC, PAT>(P, c);
+    }
+}
+
+template <typename S>
+hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
+{
+    dim3 grid(P.nblocks), blk(block);
+#define IMX_L(B, PT) hipLaunchKernelGGL((k_fused<S, B, PT>), grid, blk, 0, st, P)
+    if (bc == BC_PERIODIC) {
+        if (pattern == PAT_PULL) IMX_L(BC_PERIODIC, PAT_PULL);
+        else if (pattern == PAT_AA_EVEN) IMX_L(BC_PERIODIC, PAT_AA_EVEN);
+        else IMX_L(BC_PERIODIC, PAT_AA_ODD);
+    } else if (bc == BC_CAVITY) {
+        if (pattern == PAT_PULL) IMX_L(BC_CAVITY, PAT_PULL);
+        else if (pattern == PAT_AA_EVEN) IMX_L(BC_CAVITY, PAT_AA_EVEN);
+        else IMX_L(BC_CAVITY, PAT_AA_ODD);
+    } else {
+        if (pattern != PAT_PULL) return hipErrorInvalidValue;
+        IMX_L(BC_CHANNEL, PAT_PULL);
+    }
+#undef IMX_L
+    return hipGetLastError();
+}
+
+}  // namespace imx

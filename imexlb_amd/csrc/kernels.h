@@ -1,0 +1,102 @@
+// Kernel parameter blocks and host-side launchers shared by the HIP
+// translation units and the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.h"
+
+namespace imx {
+
+enum Bc : int { BC_PERIODIC = 0, BC_CAVITY = 1, BC_CHANNEL = 2 };
+enum Pattern : int { PAT_PULL = 0, PAT_AA_EVEN = 1, PAT_AA_ODD = 2 };
+enum Dtype : int { DT_F64 = 0, DT_F32 = 1, DT_F16 = 2 };
+
+// Division by a run-time invariant 32-bit divisor (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
+// exact for every 32-bit dividend.
+struct FastDiv {
+    uint32_t d, m, s1, s2;
+    void init(uint32_t div)
+    {
+        d = div;
+        uint32_t l = 0;
+        while ((uint64_t(1) << l) < div) ++l;
+        m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << l) - div)) / div + 1);
+        s1 = l < 1 ? l : 1;
+        s2 = l > 0 ? l - 1 : 0;
+    }
+#if defined(__HIPCC__)
+    __device__ __forceinline__ uint32_t div(uint32_t n) const
+    {
+        uint32_t t = __umulhi(m, n);
+        return (t + ((n - t) >> s1)) >> s2;
+    }
+#endif
+};
+
+// One fused stream + moments + collide launch over a single domain.
+struct FusedParams {
+    int nx, ny, nz;
+    uint32_t ncell;          // nx * ny * nz
+    uint64_t pstride;        // elements between consecutive populations
+    const void *src;         // pull: lattice read; AA: the single lattice
+    void *dst;               // pull: lattice written; AA: same as src
+    double *mom_p, *mom_u, *mom_v, *mom_w;  // written when mom_p != nullptr
+    const uint8_t *mask;     // channel: 0 fluid, 1 solid, 2 fluid next to solid
+    double tau0;             // f <- f - (f - feq) / (tau0 + 0.5)
+    double shift;            // reduced precision stores g = f - w_i * shift
+    double corr[Q];          // cavity: moving-lid term for a link crossing z = nz
+    double feq_in[Q];        // channel: inlet equilibrium, stored convention
+    uint32_t nblocks;        // grid size (for the XCD-aware block order)
+    int xcd_remap;           // 1: give each XCD one contiguous run of tiles
+    int cpt;                 // cells per thread (1, 2 or 4), along the tile
+    FastDiv divx, divy;
+};
+
+// Launchers (defined next to their kernels).
+hipError_t launch_fused(int dtype, int bc, int pattern, const FusedParams &P, int block, hipStream_t st);
+
+struct InitParams {
+    int nx, ny, nz;
+    uint32_t ncell;
+    uint64_t pstride;
+    void *dst;               // lattice written (stored convention)
+    int layout;              // 0 natural (pull), 1 swapped (AA after an even step)
+    int kind;                // 0 TGV from the per-axis tables, 1 uniform (u0,0,0)/p0, masked
+    const double *tab;       // TGV: 9 tables [sx cx c2x | sy cy c2y | sz cz c2z]
+    double u0, rho0, cs2, p0;
+    double shift;
+    const uint8_t *mask;
+    FastDiv divx, divy;
+};
+hipError_t launch_init(int dtype, const InitParams &P, hipStream_t st);
+
+// Copy between the stored lattice (any dtype, pull or AA layout after step
+// count parity) and a natural-order FP64 array of post-collision populations.
+struct XferParams {
+    int nx, ny, nz;
+    uint32_t ncell;
+    uint64_t pstride;
+    void *lat;
+    double *buf;             // [Q][nz][ny][nx]
+    int layout;              // 0 natural, 1 swapped (AA even), 2 streamed (AA odd)
+    int bc;                  // periodic or cavity (AA streamed layout needs the rule)
+    double shift;
+    double corr[Q];
+    FastDiv divx, divy;
+};
+hipError_t launch_get_f(int dtype, const XferParams &P, hipStream_t st);
+hipError_t launch_set_f(int dtype, const XferParams &P, hipStream_t st);
+
+// Deterministic FP64 reductions.
+hipError_t launch_diag(const double *p, const double *u, const double *v, const double *w,
+                       const uint8_t *mask, uint32_t n, double *partials, int nparts,
+                       double *out2, hipStream_t st);
+hipError_t launch_force(int dtype, const FusedParams &P, double *partials, int nparts,
+                        double *out3, hipStream_t st);
+
+hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st);
+
+}  // namespace imx

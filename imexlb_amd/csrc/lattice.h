@@ -1,0 +1,40 @@
+// D3Q27 lattice tables, shared by host and device code.
+//
+// Order, weights and bounce-back partners are the reference's
+// (tgv/src/lbm.cpp:27-53 weights t, :55-81 bb, :84-190 velocities e); the same
+// table appears in lid_driven_cavity/src/lbm.cpp and sphere/src/lbm.cpp.  The
+// order has to match so that population i means the same thing on both sides
+// of the C-ABI (imexlb_get_f / imexlb_set_f exchange populations by index).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define IMX_HD __host__ __device__
+#else
+#define IMX_HD
+#endif
+
+namespace imx {
+
+constexpr int Q = 27;
+constexpr int GHOST = 3;            // reference ghost depth (tgv/src/lbm.hpp:22)
+constexpr double PI_REF = 3.1415926; // reference pi (tgv/src/lbm.cpp:5)
+
+constexpr int EX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+constexpr int EY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1, -1, 1, -1, 1};
+constexpr int BB[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17, 20, 19, 22, 21, 24, 23, 26, 25};
+
+// Weight class of direction i: 0 rest, 1 face, 2 edge, 3 corner.
+constexpr int WCLASS[Q] = {0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3};
+constexpr double WVAL[4] = {8.0 / 27.0, 2.0 / 27.0, 1.0 / 54.0, 1.0 / 216.0};
+
+IMX_HD constexpr double w(int i) { return WVAL[WCLASS[i]]; }
+
+// FP16 storage keeps (f_i - w_i S) * 2^k with 2^k nearest to 1/w_i, so every
+// direction's stored value is O(|e.u|) and far from the FP16 subnormals.
+constexpr float H_SCALE[4] = {4.0f, 16.0f, 64.0f, 256.0f};
+constexpr float H_INV_SCALE[4] = {0.25f, 0.0625f, 0.015625f, 0.00390625f};
+
+}  // namespace imx

@@ -1,0 +1,171 @@
+"""Python mirror of the reference's host interface over the C-ABI.
+
+``LBM`` keeps the reference's method names and call order
+(tgv/src/lbm.hpp:238-251, driven as in tgv/main.cpp:70-119); ``System``
+parses ``input.in`` as tgv/src/System.cpp:5-24 does (per case);
+``comm_layout`` is CommHelper (tgv/src/lbm.hpp:34-117).  Every call goes to
+libimexlb.so; a failing status raises ``ImexlbError``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import ImexlbError, as_f64, dptr, lib
+
+CASES = {"tgv": L.CASE_TGV, "cavity": L.CASE_CAVITY, "ldc": L.CASE_CAVITY, "sphere": L.CASE_SPHERE}
+MODES = {"faithful": L.MODE_FAITHFUL, "config": L.MODE_CONFIG}
+ENGINES = {"fused": L.ENGINE_FUSED, "phased": L.ENGINE_PHASED}
+DTYPES = {"f64": L.F64, "f32": L.F32, "f16": L.F16}
+STORAGE = {"pull": L.PULL, "aa": L.AA}
+
+
+def _code(table, v):
+    return table[v] if isinstance(v, str) else int(v)
+
+
+def check(status, handle=None):
+    if status != L.OK:
+        msg = lib().imexlb_last_error(handle)
+        msg = msg.decode() if msg else lib().imexlb_strerror(status).decode()
+        raise ImexlbError(status, msg)
+
+
+class LBM:
+    """One subdomain (one rank) of the reference's struct LBM, on the GPU."""
+
+    def __init__(self, sx, sy, sz, tau, rho0=1.0, u0=0.05, case="tgv", mode="faithful",
+                 engine="fused", dtype="f64", storage="pull", lid_u=0.1, sphere=None,
+                 seed=5374857, block=0, cells_per_thread=0, xcd_remap=-1, device=0):
+        c = L.Config()
+        check(lib().imexlb_config_init(ctypes.byref(c)))
+        c.case_id = _code(CASES, case)
+        c.mode = _code(MODES, mode)
+        c.engine = _code(ENGINES, engine)
+        c.dtype = _code(DTYPES, dtype)
+        c.storage = _code(STORAGE, storage)
+        c.nx, c.ny, c.nz = int(sx), int(sy), int(sz)
+        c.tau0, c.rho0, c.u0, c.lid_u = float(tau), float(rho0), float(u0), float(lid_u)
+        if sphere is not None:
+            c.sphere_cx, c.sphere_cy, c.sphere_cz, c.sphere_r = (float(v) for v in sphere)
+        c.seed = int(seed)
+        c.block, c.cells_per_thread, c.xcd_remap, c.device = int(block), int(cells_per_thread), \
+            int(xcd_remap), int(device)
+        self.cfg = c
+        self.shape = (c.nz, c.ny, c.nx)
+        h = ctypes.c_void_p()
+        check(lib().imexlb_create(ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().imexlb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, s):
+        check(s, self.h)
+
+    # ---- the reference's surface (tgv/src/lbm.hpp:238-246)
+    def Initialize(self):
+        self._chk(lib().imexlb_initialize(self.h))
+
+    def setup_subdomain(self):
+        self._chk(lib().imexlb_setup_subdomain(self.h))
+
+    def Collision(self):
+        self._chk(lib().imexlb_collide(self.h))
+
+    def pack(self):
+        self._chk(lib().imexlb_pack(self.h))
+
+    def exchange(self):
+        self._chk(lib().imexlb_exchange(self.h))
+
+    def unpack(self):
+        self._chk(lib().imexlb_unpack(self.h))
+
+    def Streaming(self):
+        self._chk(lib().imexlb_stream(self.h))
+
+    def Update(self):
+        self._chk(lib().imexlb_update(self.h))
+
+    # ---- fast path and data access
+    def step(self, n, moments_every=None):
+        if moments_every is None:
+            moments_every = max(1, int(n))
+        self._chk(lib().imexlb_step(self.h, int(n), int(moments_every)))
+
+    def step_timed(self, n, moments_every=0):
+        ms = ctypes.c_float()
+        self._chk(lib().imexlb_step_timed(self.h, int(n), int(moments_every), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        self._chk(lib().imexlb_sync(self.h))
+
+    def moments(self):
+        out = [np.empty(self.shape) for _ in range(4)]
+        self._chk(lib().imexlb_get_moments(self.h, *[dptr(a) for a in out]))
+        return out
+
+    def get_f(self):
+        out = np.empty((L.Q,) + self.shape)
+        self._chk(lib().imexlb_get_f(self.h, dptr(out)))
+        return out
+
+    def set_f(self, f):
+        f = as_f64(f)
+        if f.shape != (L.Q,) + self.shape:
+            raise ValueError(f"set_f expects shape {(L.Q,) + self.shape}")
+        self._chk(lib().imexlb_set_f(self.h, dptr(f)))
+
+    def diagnostics(self):
+        ke, mass = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().imexlb_diagnostics(self.h, ctypes.byref(ke), ctypes.byref(mass)))
+        return ke.value, mass.value
+
+    def sphere_force(self):
+        out = np.zeros(3)
+        self._chk(lib().imexlb_sphere_force(self.h, dptr(out)))
+        return out
+
+    def info(self):
+        i = L.Info()
+        self._chk(lib().imexlb_get_info(self.h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in L.Info._fields_}
+
+
+class System:
+    """input.in parser (tgv/src/System.cpp:5-24 and the per-case copies)."""
+
+    def __init__(self, path="input.in", case="tgv", keep_hardcoded=True):
+        s = L.SystemC()
+        check(lib().imexlb_system_read(str(path).encode(), _code(CASES, case), int(bool(keep_hardcoded)),
+                                       ctypes.byref(s)))
+        for k, _ in L.SystemC._fields_:
+            setattr(self, k, getattr(s, k))
+
+
+def comm_layout(nranks, me, periodic, glx, gly, glz):
+    c = L.Comm()
+    check(lib().imexlb_comm_layout(int(nranks), int(me), int(bool(periodic)), int(glx), int(gly), int(glz),
+                                   ctypes.byref(c)))
+    return {"nranks": c.nranks, "me": c.me, "rx": c.rx, "ry": c.ry, "rz": c.rz, "px": c.px, "py": c.py,
+            "pz": c.pz, "neighbours": list(c.neighbours), "l_l": list(c.l_l), "lo": list(c.lo),
+            "hi": list(c.hi)}
+
+
+def copy_bandwidth(nbytes, reps=20, device=0):
+    g, ms = ctypes.c_double(), ctypes.c_float()
+    check(lib().imexlb_copy_bandwidth(int(device), int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(ms)))
+    return g.value, ms.value

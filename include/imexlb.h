@@ -1,0 +1,184 @@
+/*
+ * imexlb.h -- C-ABI of the MI355X D3Q27 BGK time step.
+ *
+ * One handle = one rank's subdomain of the reference's struct LBM
+ * (tgv/src/lbm.hpp:119-252).  The entry points replace the reference's
+ * methods one for one; the reference's driver loop
+ *   Collision -> pack -> exchange -> unpack -> Streaming -> Update
+ * (tgv/main.cpp:70-119) maps onto them unchanged.  Plain C types only: device
+ * memory is owned by the handle, host data moves through caller-owned arrays.
+ *
+ * Every function returns an int status (IMEXLB_OK = 0, negative on error);
+ * imexlb_last_error() gives the message of the last failure on a handle.
+ * A handle is not thread safe; calls on one handle enqueue in order on the
+ * handle's own HIP stream.
+ */
+#ifndef IMEXLB_H
+#define IMEXLB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMEXLB_Q 27
+#define IMEXLB_VERSION 1
+
+/* status codes */
+#define IMEXLB_OK 0
+#define IMEXLB_EINVAL (-1)       /* bad argument or configuration */
+#define IMEXLB_EHIP (-2)         /* HIP runtime error */
+#define IMEXLB_ENOMEM (-3)       /* device or host allocation failed */
+#define IMEXLB_EUNSUPPORTED (-4) /* valid, but not offered by this engine */
+#define IMEXLB_ESTATE (-5)       /* call out of order (e.g. before Initialize) */
+#define IMEXLB_EIO (-6)          /* input.in missing or malformed */
+#define IMEXLB_ENODEV (-7)       /* no HIP device */
+
+/* flow cases: the reference's three applications */
+#define IMEXLB_CASE_TGV 0    /* tgv/ : periodic Taylor-Green vortex */
+#define IMEXLB_CASE_CAVITY 1 /* lid_driven_cavity/ */
+#define IMEXLB_CASE_SPHERE 2 /* sphere/ : channel with a voxel sphere */
+
+/* boundary semantics */
+#define IMEXLB_MODE_FAITHFUL 0 /* the reference's own boundary code */
+#define IMEXLB_MODE_CONFIG 1   /* halfway bounce-back, lid momentum term,
+                                  equilibrium inlet, zero-gradient outlet */
+
+/* engines */
+#define IMEXLB_ENGINE_FUSED 0  /* one kernel per step: stream+moments+collide */
+#define IMEXLB_ENGINE_PHASED 1 /* the reference's phases as separate kernels on
+                                  its ghost-3 layout (FP64, all cases, faithful) */
+
+/* population storage */
+#define IMEXLB_F64 0 /* FP64, FP64 collision: drop-in */
+#define IMEXLB_F32 1 /* FP32 of f - w_i S, FP32 collision */
+#define IMEXLB_F16 2 /* FP16 of (f - w_i S) * 2^k_i, FP32 collision */
+
+#define IMEXLB_PULL 0 /* two lattices, pull streaming */
+#define IMEXLB_AA 1   /* one lattice, AA pattern in place */
+
+typedef struct imexlb_config {
+    int32_t case_id;  /* IMEXLB_CASE_* */
+    int32_t mode;     /* IMEXLB_MODE_* */
+    int32_t engine;   /* IMEXLB_ENGINE_* */
+    int32_t dtype;    /* IMEXLB_F64 / F32 / F16 */
+    int32_t storage;  /* IMEXLB_PULL / IMEXLB_AA */
+    int32_t nx, ny, nz; /* cells of this subdomain (the reference's glx, gly, glz) */
+    double tau0;      /* System::tau; BGK rate 1 / (tau0 + 0.5) */
+    double rho0;      /* System::rho0 */
+    double u0;        /* System::u0: TGV amplitude, channel inlet speed */
+    double lid_u;     /* config-mode cavity lid speed (faithful uses 0.1) */
+    double sphere_cx, sphere_cy, sphere_cz, sphere_r; /* config-mode sphere */
+    uint64_t seed;    /* faithful-mode sphere inlet perturbation */
+    int32_t block;    /* threads per workgroup, 0 = default */
+    int32_t cells_per_thread; /* 0 = default */
+    int32_t xcd_remap;        /* -1 = default, 0 off, 1 on */
+    int32_t device;           /* HIP device ordinal */
+} imexlb_config;
+
+typedef struct imexlb_handle imexlb_handle;
+
+typedef struct imexlb_info {
+    int64_t cells;          /* interior cells */
+    int64_t steps;          /* steps taken since Initialize */
+    double bytes_per_lup;   /* algorithmic HBM bytes per cell update */
+    int64_t device_bytes;   /* device memory held by the handle */
+    int32_t block, cells_per_thread, xcd_remap;
+} imexlb_info;
+
+/* Defaults: TGV, faithful, fused, FP64, pull, 64^3, tau0 = 0.0153, rho0 = 1,
+ * u0 = 0.05, lid 0.1, seed 5374857 (sphere/src/lbm.cpp:223). */
+int imexlb_config_init(imexlb_config *cfg);
+
+/* LBM::LBM(MPI_Comm, sx, sy, sz, tau, rho0, u0) on one rank
+ * (tgv/src/lbm.hpp:161-236): validates and allocates. */
+int imexlb_create(const imexlb_config *cfg, imexlb_handle **out);
+int imexlb_destroy(imexlb_handle *h);
+
+/* LBM::Initialize (tgv/src/lbm.cpp:7-221; lid_driven_cavity/src/lbm.cpp:
+ * 3-213; sphere/src/lbm.cpp:36-255).  May be called again to reset. */
+int imexlb_initialize(imexlb_handle *h);
+/* LBM::setup_subdomain (tgv/src/mpi_view_transfer.cpp:2-115): nothing to
+ * stage on one GPU; kept for the call sequence. */
+int imexlb_setup_subdomain(imexlb_handle *h);
+
+/* The per-phase surface (tgv/src/lbm.hpp:238-246).
+ *   fused engine : Collision, pack, exchange, unpack are ordering markers;
+ *                  Streaming runs the fused step and writes the moments;
+ *                  Update publishes them.
+ *   phased engine: each is its own kernel(s), as in the reference. */
+int imexlb_collide(imexlb_handle *h);  /* LBM::Collision, tgv/src/lbm.cpp:222 */
+int imexlb_pack(imexlb_handle *h);     /* LBM::pack, tgv/src/mpi_view_transfer.cpp:117 */
+int imexlb_exchange(imexlb_handle *h); /* LBM::exchange, :176 */
+int imexlb_unpack(imexlb_handle *h);   /* LBM::unpack, :314 */
+int imexlb_stream(imexlb_handle *h);   /* LBM::Streaming, tgv/src/lbm.cpp:239 */
+int imexlb_update(imexlb_handle *h);   /* LBM::Update, tgv/src/lbm.cpp:288 */
+
+/* n whole steps (the six calls above, n times).  Moments are written on every
+ * step whose index is a multiple of moments_every (> 0) and on the last step
+ * of the call; moments_every = 0 writes none. */
+int imexlb_step(imexlb_handle *h, int nsteps, int moments_every);
+/* As imexlb_step, bracketed by HIP events on the handle's stream; *ms is the
+ * device time of the n launches. */
+int imexlb_step_timed(imexlb_handle *h, int nsteps, int moments_every, float *ms);
+
+/* Moments p, u, v, w (the reference's p, ua, va, wa) of the last step that
+ * wrote them, interior only, [z][y][x] with x fastest.  NULL skips a field. */
+int imexlb_get_moments(imexlb_handle *h, double *p, double *u, double *v, double *w);
+/* Populations, interior, [i][z][y][x], FP64.  Fused engine: post-collision
+ * populations of the last step (the engine's state; set_f restores it).
+ * Phased engine: the reference's f array at this point of the step. */
+int imexlb_get_f(imexlb_handle *h, double *f);
+int imexlb_set_f(imexlb_handle *h, const double *f);
+
+/* FP64 kinetic energy 0.5 sum |u|^2 and mass sum 3p over the fluid cells of
+ * the last moments written. */
+int imexlb_diagnostics(imexlb_handle *h, double *ke, double *mass);
+/* Config-mode sphere: momentum-exchange force on the solid from the current
+ * post-collision populations (drag proxy), FP64. */
+int imexlb_sphere_force(imexlb_handle *h, double *force3);
+
+int imexlb_sync(imexlb_handle *h);
+int imexlb_get_info(imexlb_handle *h, imexlb_info *info);
+const char *imexlb_last_error(const imexlb_handle *h);
+const char *imexlb_strerror(int status);
+
+/* Device copy of `bytes` bytes (read + write), `reps` times: measured
+ * bandwidth ceiling (GB/s counted as read + write bytes / time). */
+int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch);
+
+/* ---- host-side pieces of the reference, no device use ----------------- */
+
+/* System (tgv/src/System.cpp:5-24, lid_driven_cavity/src/System.cpp:5-20,
+ * sphere/src/System.cpp:5-23): parse ./input.in, three whitespace-separated
+ * lines "rho R Re / u0 Time inter / nx ny nz", and derive tau per case.
+ * keep_hardcoded != 0 keeps TGV's Time = 100, inter = 10 override
+ * (tgv/src/System.cpp:17-18); 0 honours the file. */
+typedef struct imexlb_system {
+    double rho0, R, Re, u0;
+    int32_t Time, inter;
+    int32_t sx, sy, sz;
+    double cs2, cs, Ma, miu, tau;
+} imexlb_system;
+int imexlb_system_read(const char *path, int case_id, int keep_hardcoded, imexlb_system *out);
+
+/* CommHelper (tgv/src/lbm.hpp:34-117; non-periodic variant
+ * lid_driven_cavity/src/lbm.hpp:33-103): rank grid and the 26 neighbours.
+ * neighbours[] order: left right down up front back | frontup frontright
+ * frontdown frontleft backup backright backdown backleft leftup rightup
+ * leftdown rightdown | frontleftup frontleftdown frontrightup frontrightdown
+ * backleftup backleftdown backrightup backrightdown.  -1 = no neighbour. */
+typedef struct imexlb_comm {
+    int32_t nranks, me, rx, ry, rz, px, py, pz;
+    int32_t neighbours[26];
+    int32_t l_l[3];          /* local extents (tgv/src/lbm.hpp:164-166) */
+    int32_t lo[3], hi[3];    /* global offsets x_lo..z_hi (:187-231) */
+} imexlb_comm;
+int imexlb_comm_layout(int nranks, int me, int periodic, int glx, int gly, int glz, imexlb_comm *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMEXLB_H */
