@@ -36,11 +36,16 @@ def tgv_tau(n, re, u0):
     return u0 * n / re / (1.0 / 3.0) / 2 / PI_R
 
 
-def run_gpu(n, steps, warmup, storage, dtype, u0, re, block=0, cpt=0):
+def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True):
     from imexlb_amd import LBM
     g = LBM(n, n, n, tgv_tau(n, re, u0), u0=u0, case="tgv", engine="fused", dtype=dtype,
-            storage=storage, block=block, cells_per_thread=cpt)
-    g.Initialize()
+            storage=storage)
+    if tune:
+        # launch shape from the in-project autotuner (re-initialises the state)
+        choice = g.autotune(os.environ.get("IMEXLB_TUNE_CACHE"), steps=8)
+        log(f"[bench] {dtype}: autotuned block {choice[0]}, cells/thread {choice[1]}, xcd order {choice[2]}")
+    else:
+        g.Initialize()
     g.step(warmup, moments_every=0)
     g.sync()
     ms = g.step_timed(steps, moments_every=0)
@@ -92,8 +97,9 @@ def cpu_baseline(budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--re", type=float, default=1600.0)
@@ -114,7 +120,7 @@ def main():
     n, K, W = a.n, a.steps, a.warmup
     cells = n ** 3
     log(f"[bench] TGV {n}^3 {a.storage} FP64: warmup {W}, timed {K}")
-    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re)
+    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune)
     glups64 = cells * K / (ms64 * 1e-3) / 1e9
     per_launch_ms = ms64 / K
     bytes_lup = info64["bytes_per_lup"]
@@ -126,7 +132,7 @@ def main():
     if not a.no_lowp:
         for dt in ("f32", "f16"):
             log(f"[bench] TGV {n}^3 {a.storage} {dt}")
-            g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re)
+            g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re, not a.no_tune)
             gl = cells * K / (ms * 1e-3) / 1e9
             m = g.moments()
             err = vel_l2(m, m64)
@@ -164,7 +170,8 @@ def main():
                    "cells": cells, "storage": a.storage, "re": a.re, "u0": a.u0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_GBS, "traffic": traffic_from_profiles(workload),
-                     "bytes_per_lup": bytes_lup, "kernel": "k_fused (FP64, periodic, "
+                     "bytes_per_lup": bytes_lup, "launch": {k: info64[k] for k in ("block", "cells_per_thread", "xcd_remap")},
+                     "kernel": "k_fused (FP64, periodic, "
                      + ("AA" if a.storage == "aa" else "pull") + ")",
                      "measured_copy_GBps": gbps_copy},
         "cpu_baseline": cpu,

@@ -9,11 +9,12 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "lib", "libimexlb.so")
+LIB_PATH = os.environ.get("IMEXLB_LIB") or os.path.join(PKG, "lib", "libimexlb.so")
 
 Q = 27
 OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ESTATE, EIO, ENODEV = 0, -1, -2, -3, -4, -5, -6, -7
 CASE_TGV, CASE_CAVITY, CASE_SPHERE = 0, 1, 2
+ENS_PERSISTENT, ENS_GRAPH, ENS_PER_DOMAIN = 0, 1, 2
 MODE_FAITHFUL, MODE_CONFIG = 0, 1
 ENGINE_FUSED, ENGINE_PHASED = 0, 1
 F64, F32, F16 = 0, 1, 2
@@ -32,6 +33,7 @@ class Config(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
         ("xcd_remap", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("stride_pad", ctypes.c_int32),
     ]
 
 
@@ -93,6 +95,18 @@ SIGNATURES = [
     ("imexlb_copy_bandwidth", _i, [_i, ctypes.c_size_t, _i, _dp, _fp]),
     ("imexlb_system_read", _i, [ctypes.c_char_p, _i, _i, ctypes.POINTER(SystemC)]),
     ("imexlb_comm_layout", _i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(Comm)]),
+    ("imexlb_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
+    ("imexlb_ensemble_generate", _i, [ctypes.c_uint64, _i, ctypes.POINTER(EnsembleDomain)]),
+    ("imexlb_ensemble_create", _i, [ctypes.POINTER(EnsembleDomain), _i, _i, _i, _i, ctypes.POINTER(_vp)]),
+    ("imexlb_ensemble_destroy", _i, [_vp]),
+    ("imexlb_ensemble_initialize", _i, [_vp]),
+    ("imexlb_ensemble_step", _i, [_vp, _i, _i, _i]),
+    ("imexlb_ensemble_step_timed", _i, [_vp, _i, _i, _i, _fp]),
+    ("imexlb_ensemble_get_moments", _i, [_vp, _i, _dp, _dp, _dp, _dp]),
+    ("imexlb_ensemble_info", _i, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    ("imexlb_ensemble_set_launch", _i, [_vp, _i, _i, _i]),
+    ("imexlb_ensemble_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
+    ("imexlb_ensemble_last_error", ctypes.c_char_p, [_vp]),
 ]
 
 _lib = None

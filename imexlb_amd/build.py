@@ -52,8 +52,11 @@ def _run(cmd):
     return out
 
 
-def build_lib(jobs=8, verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build_lib(jobs=8, verbose=False, out=None, defines=()):
+    """Compile every HIP/C++ source and link the library (to `out`, default
+    imexlb_amd/lib/libimexlb.so); `defines` are extra -D flags."""
+    build = BUILD if out is None else out + ".objs"
+    os.makedirs(build, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     cmds = []
     objs = []
@@ -61,9 +64,9 @@ def build_lib(jobs=8, verbose=False):
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        obj = os.path.join(BUILD, src + ".o")
+        obj = os.path.join(build, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmds.append([HIPCC] + COMMON + extra + lang + ["-c", path, "-o", obj])
+        cmds.append([HIPCC] + COMMON + extra + [f"-D{d}" for d in defines] + lang + ["-c", path, "-o", obj])
         objs.append(obj)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         outs = list(ex.map(_run, cmds))
@@ -71,10 +74,11 @@ def build_lib(jobs=8, verbose=False):
         for o in outs:
             if o.strip():
                 print(o)
-    tmp = LIB + ".tmp"
+    target = LIB if out is None else out
+    tmp = target + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, target)
+    return target
 
 
 def build_driver():

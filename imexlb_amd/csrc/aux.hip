@@ -293,22 +293,21 @@ hipError_t launch_force(int dtype, const FusedParams &P, double *partials, int n
     return hipGetLastError();
 }
 
-// Plain streaming copy, 16 B per lane per access: the reference point for the
-// hardware counters and the measured bandwidth ceiling.
+// Plain copy, one 16-byte vector per lane, one pass: the reference point for
+// the hardware counters and the measured bandwidth ceiling.
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 __global__ void __launch_bounds__(256) k_copy(const dvec2 *__restrict__ src, dvec2 *__restrict__ dst, size_t n2)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n2; i += stride) __builtin_nontemporal_store(src[i], dst + i);
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n2) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st)
 {
     const size_t n2 = n / 2;
-    const unsigned blocks = 256 * 8 * 4;
-    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, (const dvec2 *)src, (dvec2 *)dst, n2);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, (const dvec2 *)src,
+                       (dvec2 *)dst, n2);
     return hipGetLastError();
 }
 

@@ -17,6 +17,10 @@
 
 using namespace imx;
 
+#ifndef IMEXLB_DEFAULT_PAD
+#define IMEXLB_DEFAULT_PAD 2304
+#endif
+
 struct imexlb_handle {
     imexlb_config cfg{};
     hipStream_t stream = nullptr;
@@ -86,6 +90,7 @@ int imexlb_config_init(imexlb_config *c)
     c->lid_u = 0.1;
     c->seed = 5374857ULL;
     c->xcd_remap = -1;
+    c->stride_pad = -1;
     return IMEXLB_OK;
 }
 
@@ -129,7 +134,9 @@ static int validate(const imexlb_config *c)
     if (!std::isfinite(c->rho0) || !std::isfinite(c->u0)) return fail(nullptr, IMEXLB_EINVAL, "non-finite rho0/u0");
     if (c->block != 0 && (c->block < 64 || c->block > 512 || c->block % 64))
         return fail(nullptr, IMEXLB_EINVAL, "block %d must be a multiple of 64 in [64, 512]", c->block);
-    if (c->cells_per_thread < 0 || c->cells_per_thread > 8)
+    if (c->stride_pad < -1 || c->stride_pad > (1 << 24) || (c->stride_pad > 0 && c->stride_pad % 256))
+        return fail(nullptr, IMEXLB_EINVAL, "stride_pad %d must be -1 or a multiple of 256 below 16 MiB", c->stride_pad);
+    if (c->cells_per_thread < 0 || c->cells_per_thread > 2)
         return fail(nullptr, IMEXLB_EINVAL, "cells_per_thread %d", c->cells_per_thread);
     if (c->engine == IMEXLB_ENGINE_FUSED) {
         if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id != IMEXLB_CASE_TGV)
@@ -163,6 +170,7 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.dst = h->lat[dst];
     P.mask = h->mask;
     P.tau0 = c.tau0;
+    P.omega_f = (float)(1.0 / (c.tau0 + 0.5));
     P.shift = h->shift;
     std::memcpy(P.corr, h->corr, sizeof P.corr);
     std::memcpy(P.feq_in, h->feq_in, sizeof P.feq_in);
@@ -264,8 +272,14 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
     }
 
     h->ncell = (uint32_t)((uint64_t)c.nx * c.ny * c.nz);
-    h->pstride = h->ncell;
     h->elem = c.dtype == IMEXLB_F64 ? 8 : (c.dtype == IMEXLB_F32 ? 4 : 2);
+    {
+        // population stride: the cell count rounded up to 256 bytes, plus a
+        // pad that staggers the 27 streams across HBM channels
+        const uint64_t per = 256 / h->elem;
+        const int pad = c.stride_pad < 0 ? IMEXLB_DEFAULT_PAD : c.stride_pad;
+        h->pstride = ((uint64_t)h->ncell + per - 1) / per * per + (uint64_t)pad / h->elem;
+    }
     h->block = c.block ? c.block : 256;
     h->cpt = c.cells_per_thread ? c.cells_per_thread : 1;
     h->xcd = c.xcd_remap < 0 ? 1 : c.xcd_remap;
@@ -658,6 +672,47 @@ int imexlb_get_info(imexlb_handle *h, imexlb_info *info)
     info->cells_per_thread = h->cpt;
     info->xcd_remap = h->xcd;
     return IMEXLB_OK;
+}
+
+int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t *chosen)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (h->ph) return fail(h, IMEXLB_EUNSUPPORTED, "the phased engine has no launch shapes to tune");
+    int rc = set_device(h);
+    if (rc) return rc;
+    const imexlb_config &c = h->cfg;
+    char shape[96];
+    snprintf(shape, sizeof shape, "bc%d_%dx%dx%d", bc_of(c), c.nx, c.ny, c.nz);
+    const std::string key = tune_key(h->device, "single", c.dtype, c.storage, shape);
+    TuneChoice best{};
+    if (!tune_cache_read(cache_path, key, best)) {
+        if (steps < 2) steps = 2;
+        steps += steps & 1;  // whole AA periods
+        if (!h->initialized && (rc = imexlb_initialize(h))) return rc;
+        best.ms = 1e30f;
+        const int blocks[4] = {64, 128, 256, 512};
+        for (int b : blocks)
+            for (int cpt = 1; cpt <= 2; ++cpt)
+                for (int xcd = 0; xcd <= 1; ++xcd) {
+                    h->block = b;
+                    h->cpt = cpt;
+                    h->xcd = xcd;
+                    if ((rc = imexlb_step(h, 2, 0))) return rc;
+                    float ms = 0;
+                    if ((rc = imexlb_step_timed(h, steps, 0, &ms))) return rc;
+                    if (ms < best.ms) best = TuneChoice{b, cpt, xcd, ms};
+                }
+        tune_cache_write(cache_path, key, best);
+    }
+    h->block = best.block;
+    h->cpt = best.cpt;
+    h->xcd = best.grid_or_xcd;
+    if (chosen) {
+        chosen[0] = best.block;
+        chosen[1] = best.cpt;
+        chosen[2] = best.grid_or_xcd;
+    }
+    return imexlb_initialize(h);
 }
 
 int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch)

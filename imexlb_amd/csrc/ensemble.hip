@@ -1,0 +1,603 @@
+// Batched ensemble: many independent domains of uneven size and Reynolds
+// number advanced by one persistent launch per step, optionally replayed from
+// a captured HIP graph.  Each domain is a TGV box (periodic) or a config-mode
+// lid-driven cavity; all share one storage type and step in lockstep.
+//
+// The work list is a table of tiles (domain, first cell); a fixed grid of
+// workgroups walks it, workgroup b taking tiles b, b + G, b + 2G, ...  Every
+// tile runs the single-domain cell code (fused.h) with that domain's
+// parameters, so an ensemble member steps bit for bit as it would alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/imexlb.h"
+#include "fused.h"
+#include "kernels.h"
+
+namespace imx {
+
+struct EnsParams {
+    const FusedParams *doms;  // two entries per domain: [2 d + parity]
+    const uint2 *tiles;       // (domain, first cell)
+    uint32_t ntiles;
+    int parity;               // pull: which lattice of each pair is read
+    int write_moments;
+};
+
+template <typename S, int BC, int PAT, int CPT>
+__device__ __forceinline__ void tile_run(const FusedParams &P, uint32_t first, bool wm)
+{
+    using R = typename RealOf<S>::type;
+    CellCtx k[CPT];
+    R f[CPT][Q];
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+        cell_setup<BC>(P, first + (uint32_t)r * blockDim.x + threadIdx.x, k[r]);
+        if (k[r].live) cell_gather<S, BC, PAT>(P, k[r], f[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+        if (!k[r].live) continue;
+        cell_collide<S>(P, k[r], f[r], wm);
+        cell_scatter<S, BC, PAT>(P, k[r], f[r]);
+    }
+}
+
+// Persistent over the tile table: every workgroup exits after its last tile.
+template <typename S, int PAT, int CPT>
+__global__ void __launch_bounds__(512) k_ensemble(const EnsParams EP)
+{
+    const bool wm = EP.write_moments != 0;
+    for (uint32_t t = blockIdx.x; t < EP.ntiles; t += gridDim.x) {
+        const uint2 tl = EP.tiles[t];
+        const FusedParams &P = EP.doms[2 * tl.x + (uint32_t)EP.parity];
+        if (P.bc == BC_PERIODIC) tile_run<S, BC_PERIODIC, PAT, CPT>(P, tl.y, wm);
+        else tile_run<S, BC_CAVITY, PAT, CPT>(P, tl.y, wm);
+    }
+}
+
+template <typename S, int CPT>
+static hipError_t launch_ens_cpt(int pattern, const EnsParams &EP, int grid, int block, hipStream_t st)
+{
+    if (pattern == PAT_PULL) hipLaunchKernelGGL((k_ensemble<S, PAT_PULL, CPT>), dim3(grid), dim3(block), 0, st, EP);
+    else if (pattern == PAT_AA_EVEN)
+        hipLaunchKernelGGL((k_ensemble<S, PAT_AA_EVEN, CPT>), dim3(grid), dim3(block), 0, st, EP);
+    else hipLaunchKernelGGL((k_ensemble<S, PAT_AA_ODD, CPT>), dim3(grid), dim3(block), 0, st, EP);
+    return hipGetLastError();
+}
+
+static hipError_t launch_ensemble(int dtype, int cpt, int pattern, const EnsParams &EP, int grid, int block,
+                                  hipStream_t st)
+{
+    if (dtype == DT_F64) return cpt == 2 ? launch_ens_cpt<double, 2>(pattern, EP, grid, block, st)
+                                         : launch_ens_cpt<double, 1>(pattern, EP, grid, block, st);
+    if (dtype == DT_F32) return cpt == 2 ? launch_ens_cpt<float, 2>(pattern, EP, grid, block, st)
+                                         : launch_ens_cpt<float, 1>(pattern, EP, grid, block, st);
+    return cpt == 2 ? launch_ens_cpt<_Float16, 2>(pattern, EP, grid, block, st)
+                    : launch_ens_cpt<_Float16, 1>(pattern, EP, grid, block, st);
+}
+
+}  // namespace imx
+
+using namespace imx;
+
+struct imexlb_ensemble {
+    int n = 0, dtype = 0, storage = 0, device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<imexlb_domain> doms;
+    std::vector<FusedParams> hp;         // host copy of the domain table
+    FusedParams *dp = nullptr;           // device domain table
+    std::vector<uint2> tiles;
+    uint2 *dtiles = nullptr;
+    std::vector<void *> lat;             // 2 per domain (second null for AA)
+    std::vector<double *> mom;           // 4 per domain
+    std::vector<double *> tabs;          // TGV init tables per domain
+    double *dummy = nullptr;
+    int block = 256, cpt = 1, grid = 0, cus = 256;
+    int64_t steps = 0, mom_step = -1, cells = 0, device_bytes = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_parity = -1;               // step parity the captured pair starts from
+    std::string err;
+};
+
+static int efail(imexlb_ensemble *e, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    static thread_local std::string g;
+    if (e) e->err = buf;
+    else g = buf;
+    return code;
+}
+
+#define ECHK(e, expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return efail((e), IMEXLB_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+static void drop_graph(imexlb_ensemble *e)
+{
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    e->graph_parity = -1;
+}
+
+static void build_tiles(imexlb_ensemble *e)
+{
+    e->tiles.clear();
+    const uint32_t tile = (uint32_t)e->block * (uint32_t)e->cpt;
+    // largest domains first, so the tail of the table is made of small work
+    std::vector<int> order(e->n);
+    for (int d = 0; d < e->n; ++d) order[d] = d;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return e->hp[2 * a].ncell > e->hp[2 * b].ncell; });
+    for (int d : order)
+        for (uint32_t c = 0; c < e->hp[2 * d].ncell; c += tile) e->tiles.push_back(make_uint2((uint32_t)d, c));
+}
+
+static int upload_tables(imexlb_ensemble *e)
+{
+    if (e->dtiles) (void)hipFree(e->dtiles);
+    e->dtiles = nullptr;
+    ECHK(e, hipMalloc(&e->dtiles, e->tiles.size() * sizeof(uint2)));
+    ECHK(e, hipMemcpy(e->dtiles, e->tiles.data(), e->tiles.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    for (auto &P : e->hp) {
+        P.cpt = e->cpt;
+        const uint32_t tile = (uint32_t)e->block * (uint32_t)e->cpt;
+        P.nblocks = (P.ncell + tile - 1) / tile;
+    }
+    ECHK(e, hipMemcpy(e->dp, e->hp.data(), e->hp.size() * sizeof(FusedParams), hipMemcpyHostToDevice));
+    drop_graph(e);
+    return IMEXLB_OK;
+}
+
+static int pattern_of(const imexlb_ensemble *e)
+{
+    if (e->storage == IMEXLB_PULL) return PAT_PULL;
+    return (e->steps % 2 == 0) ? PAT_AA_ODD : PAT_AA_EVEN;
+}
+
+static int launch_persistent(imexlb_ensemble *e, int pattern, int parity, bool wm)
+{
+    EnsParams EP;
+    EP.doms = e->dp;
+    EP.tiles = e->dtiles;
+    EP.ntiles = (uint32_t)e->tiles.size();
+    EP.parity = parity;
+    EP.write_moments = wm ? 1 : 0;
+    const int grid = std::min<int>(e->grid, (int)e->tiles.size());
+    ECHK(e, launch_ensemble(e->dtype, e->cpt, pattern, EP, grid, e->block, e->stream));
+    return IMEXLB_OK;
+}
+
+// one step of every domain; per_domain = one single-domain launch each
+static int ens_step(imexlb_ensemble *e, bool per_domain, bool wm)
+{
+    const int pattern = pattern_of(e);
+    const int parity = e->storage == IMEXLB_PULL ? (int)(e->steps & 1) : 0;
+    if (per_domain) {
+        for (int d = 0; d < e->n; ++d) {
+            FusedParams P = e->hp[2 * d + parity];
+            P.xcd_remap = 0;
+            if (!wm) P.mom_p = P.mom_u = P.mom_v = P.mom_w = nullptr;
+            ECHK(e, launch_fused(e->dtype, P.bc, pattern, P, e->block, e->stream));
+        }
+    } else {
+        int rc = launch_persistent(e, pattern, parity, wm);
+        if (rc) return rc;
+    }
+    e->steps++;
+    if (wm) e->mom_step = e->steps;
+    return IMEXLB_OK;
+}
+
+// capture two consecutive steps (a full period of the pull parity and of the
+// AA odd/even pair) starting from the current parity
+static int capture_pair(imexlb_ensemble *e)
+{
+    drop_graph(e);
+    const int64_t s0 = e->steps;
+    ECHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ens_step(e, false, false);
+    if (!rc) rc = ens_step(e, false, false);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    e->steps = s0;  // capture recorded the launches; nothing ran
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return efail(e, IMEXLB_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    e->graph = g;
+    ECHK(e, hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+    e->graph_parity = (int)(s0 & 1);
+    return IMEXLB_OK;
+}
+
+extern "C" {
+
+// The ensemble of the batched benchmark: n domains from a 64-bit seed, half
+// TGV and half config-mode cavity (alternating), edge from {32,48,64,80,96},
+// Re uniform in [50, 1000], lid speed uniform in [0.05, 0.1], TGV U0 = 0.05.
+// Drawn with std::mt19937_64 and explicit 53-bit scaling so every platform
+// draws the same domains.
+int imexlb_ensemble_generate(uint64_t seed, int n, imexlb_domain *out)
+{
+    if (n < 1 || !out) return IMEXLB_EINVAL;
+    std::mt19937_64 rng(seed);
+    auto unif = [&]() { return (double)(rng() >> 11) * (1.0 / 9007199254740992.0); };
+    static const int edges[5] = {32, 48, 64, 80, 96};
+    for (int d = 0; d < n; ++d) {
+        imexlb_domain &D = out[d];
+        std::memset(&D, 0, sizeof D);
+        D.case_id = (d % 2 == 0) ? IMEXLB_CASE_TGV : IMEXLB_CASE_CAVITY;
+        const int edge = edges[rng() % 5];
+        D.nx = D.ny = D.nz = edge;
+        const double re = 50.0 + 950.0 * unif();
+        const double lid = 0.05 + 0.05 * unif();
+        D.rho0 = 1.0;
+        if (D.case_id == IMEXLB_CASE_TGV) {
+            D.u0 = 0.05;
+            D.lid_u = 0.0;
+            D.tau0 = D.u0 * edge / re / (1.0 / 3.0) / 2 / 3.1415926;  // tgv/src/System.cpp:21
+        } else {
+            D.u0 = 0.0;
+            D.lid_u = lid;
+            D.tau0 = lid * edge / re / (1.0 / 3.0);  // lid_driven_cavity/src/System.cpp:19, R = edge
+        }
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_ensemble_destroy(imexlb_ensemble *e)
+{
+    if (!e) return IMEXLB_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    for (void *p : e->lat)
+        if (p) (void)hipFree(p);
+    for (double *p : e->mom)
+        if (p) (void)hipFree(p);
+    for (double *p : e->tabs)
+        if (p) (void)hipFree(p);
+    if (e->dp) (void)hipFree(e->dp);
+    if (e->dtiles) (void)hipFree(e->dtiles);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return IMEXLB_OK;
+}
+
+int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int storage, int device,
+                           imexlb_ensemble **out)
+{
+    if (!doms || n < 1 || !out || dtype < 0 || dtype > 2 || storage < 0 || storage > 1) return IMEXLB_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return IMEXLB_ENODEV;
+    if (device < 0 || device >= ndev) return IMEXLB_EINVAL;
+    for (int d = 0; d < n; ++d) {
+        const imexlb_domain &D = doms[d];
+        if ((D.case_id != IMEXLB_CASE_TGV && D.case_id != IMEXLB_CASE_CAVITY) || D.nx < 2 || D.ny < 2 || D.nz < 2 ||
+            (uint64_t)D.nx * D.ny * D.nz >= (1ull << 31) || !(D.tau0 > -0.5))
+            return IMEXLB_EINVAL;
+    }
+    imexlb_ensemble *e = new imexlb_ensemble;
+    e->n = n;
+    e->dtype = dtype;
+    e->storage = storage;
+    e->device = device;
+    e->doms.assign(doms, doms + n);
+    auto bail = [&](int code) {
+        imexlb_ensemble_destroy(e);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess) return bail(IMEXLB_EHIP);
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) e->cus = cus;
+    e->grid = e->cus * 8;
+    const size_t elem = dtype == IMEXLB_F64 ? 8 : (dtype == IMEXLB_F32 ? 4 : 2);
+    e->hp.resize(2 * (size_t)n);
+    e->lat.assign(2 * (size_t)n, nullptr);
+    e->mom.assign(4 * (size_t)n, nullptr);
+    e->tabs.assign((size_t)n, nullptr);
+    for (int d = 0; d < n; ++d) {
+        const imexlb_domain &D = doms[d];
+        const uint32_t nc = (uint32_t)((uint64_t)D.nx * D.ny * D.nz);
+        e->cells += nc;
+        const size_t lb = (size_t)Q * nc * elem;
+        const int nl = storage == IMEXLB_PULL ? 2 : 1;
+        for (int k = 0; k < nl; ++k) {
+            if (hipMalloc(&e->lat[2 * d + k], lb) != hipSuccess) return bail(IMEXLB_ENOMEM);
+            e->device_bytes += lb;
+        }
+        for (int k = 0; k < 4; ++k) {
+            if (hipMalloc(&e->mom[4 * d + k], (size_t)nc * 8) != hipSuccess) return bail(IMEXLB_ENOMEM);
+            e->device_bytes += (int64_t)nc * 8;
+        }
+        for (int par = 0; par < 2; ++par) {
+            FusedParams &P = e->hp[2 * d + par];
+            std::memset(&P, 0, sizeof P);
+            P.nx = D.nx;
+            P.ny = D.ny;
+            P.nz = D.nz;
+            P.ncell = nc;
+            P.pstride = nc;
+            if (storage == IMEXLB_PULL) {
+                P.src = e->lat[2 * d + par];
+                P.dst = e->lat[2 * d + 1 - par];
+            } else {
+                P.src = P.dst = e->lat[2 * d];
+            }
+            P.mom_p = e->mom[4 * d];
+            P.mom_u = e->mom[4 * d + 1];
+            P.mom_v = e->mom[4 * d + 2];
+            P.mom_w = e->mom[4 * d + 3];
+            P.tau0 = D.tau0;
+            P.omega_f = (float)(1.0 / (D.tau0 + 0.5));
+            P.shift = dtype == IMEXLB_F64 ? 0.0 : D.rho0;
+            P.bc = D.case_id == IMEXLB_CASE_TGV ? BC_PERIODIC : BC_CAVITY;
+            if (D.case_id == IMEXLB_CASE_CAVITY)
+                for (int i = 0; i < Q; ++i) P.corr[i] = 6.0 * w(i) * D.rho0 * ((double)EX[i] * D.lid_u);
+            P.divx.init(D.nx);
+            P.divy.init(D.ny);
+        }
+        if (D.case_id == IMEXLB_CASE_TGV) {
+            std::vector<double> tab(3 * ((size_t)D.nx + D.ny + D.nz));
+            size_t o = 0;
+            const int ext[3] = {D.nx, D.ny, D.nz};
+            for (int a = 0; a < 3; ++a) {
+                const int m = ext[a];
+                for (int k = 0; k < m; ++k) tab[o + k] = std::sin((double)k / (double)m * 2.0 * PI_REF);
+                for (int k = 0; k < m; ++k) tab[o + m + k] = std::cos((double)k / (double)m * 2.0 * PI_REF);
+                for (int k = 0; k < m; ++k) tab[o + 2 * m + k] = std::cos((double)k / (double)m * 2.0 * PI_REF * 2.0);
+                o += 3 * (size_t)m;
+            }
+            if (hipMalloc(&e->tabs[d], tab.size() * 8) != hipSuccess) return bail(IMEXLB_ENOMEM);
+            if (hipMemcpy(e->tabs[d], tab.data(), tab.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+                return bail(IMEXLB_EHIP);
+        }
+    }
+    if (hipMalloc(&e->dp, e->hp.size() * sizeof(FusedParams)) != hipSuccess) return bail(IMEXLB_ENOMEM);
+    build_tiles(e);
+    if (upload_tables(e)) return bail(IMEXLB_EHIP);
+    *out = e;
+    return IMEXLB_OK;
+}
+
+int imexlb_ensemble_initialize(imexlb_ensemble *e)
+{
+    if (!e) return IMEXLB_EINVAL;
+    ECHK(e, hipSetDevice(e->device));
+    for (int d = 0; d < e->n; ++d) {
+        const imexlb_domain &D = e->doms[d];
+        const FusedParams &F = e->hp[2 * d];
+        InitParams P;
+        std::memset(&P, 0, sizeof P);
+        P.nx = D.nx;
+        P.ny = D.ny;
+        P.nz = D.nz;
+        P.ncell = F.ncell;
+        P.pstride = F.pstride;
+        P.dst = e->lat[2 * d];
+        P.layout = e->storage == IMEXLB_AA ? 1 : 0;
+        P.kind = D.case_id == IMEXLB_CASE_TGV ? 0 : 1;
+        P.tab = e->tabs[d];
+        P.u0 = D.case_id == IMEXLB_CASE_TGV ? D.u0 : 0.0;
+        P.rho0 = D.rho0;
+        P.cs2 = 1.0 / 3.0;
+        P.p0 = D.rho0 * P.cs2;
+        P.shift = F.shift;
+        P.divx.init(D.nx);
+        P.divy.init(D.ny);
+        ECHK(e, launch_init(e->dtype, P, e->stream));
+    }
+    ECHK(e, hipStreamSynchronize(e->stream));
+    e->steps = 0;
+    e->mom_step = -1;
+    return IMEXLB_OK;
+}
+
+// mode 0: persistent launch per step; 1: persistent launches replayed from a
+// captured graph; 2: one single-domain launch per domain per step.
+int imexlb_ensemble_step(imexlb_ensemble *e, int nsteps, int mode, int moments_every)
+{
+    if (!e || nsteps < 0 || mode < 0 || mode > 2 || moments_every < 0) return IMEXLB_EINVAL;
+    ECHK(e, hipSetDevice(e->device));
+    int rc;
+    int k = 0;
+    while (k < nsteps) {
+        const int64_t idx = e->steps + 1;
+        const bool last = k == nsteps - 1;
+        const bool wm = moments_every > 0 && (idx % moments_every == 0 || last);
+        if (mode == 1 && !wm && nsteps - k >= 2) {
+            // graph pairs until two steps before the next moments step
+            const bool wm2 = moments_every > 0 && ((idx + 1) % moments_every == 0 || k + 1 == nsteps - 1);
+            if (!wm2) {
+                if (e->graph_parity != (int)(e->steps & 1) || !e->gexec) {
+                    if ((rc = capture_pair(e))) return rc;
+                }
+                ECHK(e, hipGraphLaunch(e->gexec, e->stream));
+                e->steps += 2;
+                k += 2;
+                continue;
+            }
+        }
+        if ((rc = ens_step(e, mode == 2, wm))) return rc;
+        ++k;
+    }
+    return IMEXLB_OK;
+}
+
+int imexlb_ensemble_step_timed(imexlb_ensemble *e, int nsteps, int mode, int moments_every, float *ms)
+{
+    if (!e || !ms) return IMEXLB_EINVAL;
+    ECHK(e, hipSetDevice(e->device));
+    // build the graph outside the timed region
+    if (mode == 1 && (e->graph_parity != (int)(e->steps & 1) || !e->gexec)) {
+        int rc = capture_pair(e);
+        if (rc) return rc;
+    }
+    hipEvent_t a, b;
+    ECHK(e, hipEventCreate(&a));
+    ECHK(e, hipEventCreate(&b));
+    ECHK(e, hipEventRecord(a, e->stream));
+    int rc = imexlb_ensemble_step(e, nsteps, mode, moments_every);
+    if (!rc) {
+        ECHK(e, hipEventRecord(b, e->stream));
+        ECHK(e, hipEventSynchronize(b));
+        ECHK(e, hipEventElapsedTime(ms, a, b));
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+int imexlb_ensemble_get_moments(imexlb_ensemble *e, int d, double *p, double *u, double *v, double *w)
+{
+    if (!e || d < 0 || d >= e->n) return IMEXLB_EINVAL;
+    if (e->mom_step != e->steps) return efail(e, IMEXLB_ESTATE, "no moments for step %lld", (long long)e->steps);
+    ECHK(e, hipSetDevice(e->device));
+    double *out[4] = {p, u, v, w};
+    const size_t nb = (size_t)e->hp[2 * d].ncell * 8;
+    for (int k = 0; k < 4; ++k)
+        if (out[k]) ECHK(e, hipMemcpyAsync(out[k], e->mom[4 * d + k], nb, hipMemcpyDeviceToHost, e->stream));
+    ECHK(e, hipStreamSynchronize(e->stream));
+    return IMEXLB_OK;
+}
+
+// out: cells, tiles, grid, block, cells per thread, device bytes, steps
+int imexlb_ensemble_info(imexlb_ensemble *e, int64_t *out)
+{
+    if (!e || !out) return IMEXLB_EINVAL;
+    out[0] = e->cells;
+    out[1] = (int64_t)e->tiles.size();
+    out[2] = std::min<int64_t>(e->grid, (int64_t)e->tiles.size());
+    out[3] = e->block;
+    out[4] = e->cpt;
+    out[5] = e->device_bytes;
+    out[6] = e->steps;
+    return IMEXLB_OK;
+}
+
+int imexlb_ensemble_set_launch(imexlb_ensemble *e, int block, int cpt, int grid)
+{
+    if (!e || block < 64 || block > 512 || block % 64 || (cpt != 1 && cpt != 2) || grid < 1) return IMEXLB_EINVAL;
+    e->block = block;
+    e->cpt = cpt;
+    e->grid = grid;
+    build_tiles(e);
+    return upload_tables(e);
+}
+
+const char *imexlb_ensemble_last_error(const imexlb_ensemble *e) { return e ? e->err.c_str() : ""; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ autotuner
+
+namespace imx {
+
+std::string tune_key(int device, const char *kind, int dtype, int storage, const std::string &shape)
+{
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) arch = prop.gcnArchName;
+    for (char &c : arch)
+        if (c == ' ' || c == '"') c = '_';
+    std::ostringstream k;
+    k << kind << "|" << arch << "|dt" << dtype << "|st" << storage << "|" << shape;
+    return k.str();
+}
+
+// JSON lines: {"key": "...", "block": B, "cpt": C, "third": T, "ms": M}
+bool tune_cache_read(const char *path, const std::string &key, TuneChoice &out)
+{
+    if (!path || !*path) return false;
+    std::ifstream in(path);
+    std::string line;
+    bool found = false;
+    while (std::getline(in, line)) {
+        const std::string pat = "\"key\": \"" + key + "\"";
+        if (line.find(pat) == std::string::npos) continue;
+        TuneChoice c{};
+        const char *s = line.c_str() + line.find(pat) + pat.size();
+        if (sscanf(s, ", \"block\": %d, \"cpt\": %d, \"third\": %d, \"ms\": %f", &c.block, &c.cpt, &c.grid_or_xcd,
+                   &c.ms) == 4) {
+            out = c;
+            found = true;  // last entry for the key wins
+        }
+    }
+    return found;
+}
+
+void tune_cache_write(const char *path, const std::string &key, const TuneChoice &c)
+{
+    if (!path || !*path) return;
+    std::ofstream o(path, std::ios::app);
+    o << "{\"key\": \"" << key << "\", \"block\": " << c.block << ", \"cpt\": " << c.cpt << ", \"third\": "
+      << c.grid_or_xcd << ", \"ms\": " << c.ms << "}\n";
+}
+
+}  // namespace imx
+
+extern "C" {
+
+// Time every launch shape (workgroup size x cells per thread x persistent
+// grid) for a few graph-replayed steps, keep the fastest, record it in the
+// JSON-lines cache at cache_path (NULL = no cache).  A cache hit skips the
+// timing.  All shapes give bit-identical results; the ensemble is reset to
+// its initial state afterwards.  chosen[3] receives block, cpt, grid.
+int imexlb_ensemble_autotune(imexlb_ensemble *e, const char *cache_path, int steps, int32_t *chosen)
+{
+    if (!e) return IMEXLB_EINVAL;
+    ECHK(e, hipSetDevice(e->device));
+    std::ostringstream shape;
+    shape << "ens" << e->n << "_cells" << e->cells;
+    const std::string key = tune_key(e->device, "ensemble", e->dtype, e->storage, shape.str());
+    TuneChoice best{};
+    if (!tune_cache_read(cache_path, key, best)) {
+        if (steps < 2) steps = 2;
+        steps += steps & 1;
+        best.ms = 1e30f;
+        const int blocks[3] = {128, 256, 512};
+        const int grids[3] = {e->cus * 4, e->cus * 8, e->cus * 16};
+        for (int b : blocks)
+            for (int c = 1; c <= 2; ++c)
+                for (int g : grids) {
+                    int rc = imexlb_ensemble_set_launch(e, b, c, g);
+                    if (rc) return rc;
+                    if ((rc = imexlb_ensemble_step(e, 2, 1, 0))) return rc;  // warm: capture + one replay
+                    float ms = 0;
+                    if ((rc = imexlb_ensemble_step_timed(e, steps, 1, 0, &ms))) return rc;
+                    if (ms < best.ms) best = TuneChoice{b, c, g, ms};
+                }
+        tune_cache_write(cache_path, key, best);
+    }
+    int rc = imexlb_ensemble_set_launch(e, best.block, best.cpt, best.grid_or_xcd);
+    if (rc) return rc;
+    if (chosen) {
+        chosen[0] = best.block;
+        chosen[1] = best.cpt;
+        chosen[2] = best.grid_or_xcd;
+    }
+    return imexlb_ensemble_initialize(e);
+}
+
+}  // extern "C"

@@ -46,12 +46,14 @@ struct FusedParams {
     double *mom_p, *mom_u, *mom_v, *mom_w;  // written when mom_p != nullptr
     const uint8_t *mask;     // channel: 0 fluid, 1 solid, 2 fluid next to solid
     double tau0;             // f <- f - (f - feq) / (tau0 + 0.5)
+    float omega_f;           // 1 / (tau0 + 0.5) for the FP32 collision
     double shift;            // reduced precision stores g = f - w_i * shift
     double corr[Q];          // cavity: moving-lid term for a link crossing z = nz
     double feq_in[Q];        // channel: inlet equilibrium, stored convention
     uint32_t nblocks;        // grid size (for the XCD-aware block order)
     int xcd_remap;           // 1: give each XCD one contiguous run of tiles
-    int cpt;                 // cells per thread (1, 2 or 4), along the tile
+    int cpt;                 // cells per thread (1 or 2), blockDim apart
+    int bc;                  // boundary kind (read by the ensemble kernel)
     FastDiv divx, divy;
 };
 
@@ -98,5 +100,20 @@ hipError_t launch_force(int dtype, const FusedParams &P, double *partials, int n
                         double *out3, hipStream_t st);
 
 hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st);
+
+}  // namespace imx
+
+#include <string>
+
+namespace imx {
+
+// Autotuner cache (JSON lines), shared by the single-domain and ensemble tuners.
+struct TuneChoice {
+    int block, cpt, grid_or_xcd;
+    float ms;
+};
+std::string tune_key(int device, const char *kind, int dtype, int storage, const std::string &shape);
+bool tune_cache_read(const char *path, const std::string &key, TuneChoice &out);
+void tune_cache_write(const char *path, const std::string &key, const TuneChoice &c);
 
 }  // namespace imx

@@ -36,7 +36,7 @@ class LBM:
 
     def __init__(self, sx, sy, sz, tau, rho0=1.0, u0=0.05, case="tgv", mode="faithful",
                  engine="fused", dtype="f64", storage="pull", lid_u=0.1, sphere=None,
-                 seed=5374857, block=0, cells_per_thread=0, xcd_remap=-1, device=0):
+                 seed=5374857, block=0, cells_per_thread=0, xcd_remap=-1, device=0, stride_pad=-1):
         c = L.Config()
         check(lib().imexlb_config_init(ctypes.byref(c)))
         c.case_id = _code(CASES, case)
@@ -51,6 +51,7 @@ class LBM:
         c.seed = int(seed)
         c.block, c.cells_per_thread, c.xcd_remap, c.device = int(block), int(cells_per_thread), \
             int(xcd_remap), int(device)
+        c.stride_pad = int(stride_pad)
         self.cfg = c
         self.shape = (c.nz, c.ny, c.nx)
         h = ctypes.c_void_p()
@@ -139,6 +140,13 @@ class LBM:
         self._chk(lib().imexlb_sphere_force(self.h, dptr(out)))
         return out
 
+    def autotune(self, cache_path=None, steps=10):
+        """Pick the fastest launch shape (results are shape-independent); the
+        handle is re-initialised afterwards.  Returns (block, cpt, xcd)."""
+        ch = (ctypes.c_int32 * 3)()
+        self._chk(lib().imexlb_autotune(self.h, cache_path.encode() if cache_path else None, int(steps), ch))
+        return tuple(ch)
+
     def info(self):
         i = L.Info()
         self._chk(lib().imexlb_get_info(self.h, ctypes.byref(i)))
@@ -169,3 +177,76 @@ def copy_bandwidth(nbytes, reps=20, device=0):
     g, ms = ctypes.c_double(), ctypes.c_float()
     check(lib().imexlb_copy_bandwidth(int(device), int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(ms)))
     return g.value, ms.value
+
+
+class Ensemble:
+    """Many independent TGV / config-mode cavity domains stepped together by
+    one persistent launch per step (optionally replayed from a HIP graph)."""
+
+    MODES = {"persistent": L.ENS_PERSISTENT, "graph": L.ENS_GRAPH, "per_domain": L.ENS_PER_DOMAIN}
+
+    def __init__(self, domains, dtype="f32", storage="pull", device=0):
+        n = len(domains)
+        arr = (L.EnsembleDomain * n)()
+        for k, d in enumerate(domains):
+            arr[k] = d if isinstance(d, L.EnsembleDomain) else L.EnsembleDomain(**d)
+        self.domains = [arr[k] for k in range(n)]
+        h = ctypes.c_void_p()
+        check(lib().imexlb_ensemble_create(arr, n, _code(DTYPES, dtype), _code(STORAGE, storage), int(device),
+                                           ctypes.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def generate(seed, n):
+        arr = (L.EnsembleDomain * n)()
+        check(lib().imexlb_ensemble_generate(int(seed), int(n), arr))
+        return [arr[k] for k in range(n)]
+
+    def _chk(self, s):
+        if s != L.OK:
+            msg = lib().imexlb_ensemble_last_error(self.h)
+            raise ImexlbError(s, (msg.decode() if msg else "") or lib().imexlb_strerror(s).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().imexlb_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def Initialize(self):
+        self._chk(lib().imexlb_ensemble_initialize(self.h))
+
+    def step(self, n, mode="graph", moments_every=None):
+        if moments_every is None:
+            moments_every = max(1, int(n))
+        self._chk(lib().imexlb_ensemble_step(self.h, int(n), self.MODES[mode], int(moments_every)))
+
+    def step_timed(self, n, mode="graph", moments_every=0):
+        ms = ctypes.c_float()
+        self._chk(lib().imexlb_ensemble_step_timed(self.h, int(n), self.MODES[mode], int(moments_every),
+                                                   ctypes.byref(ms)))
+        return ms.value
+
+    def moments(self, d):
+        dom = self.domains[d]
+        shape = (dom.nz, dom.ny, dom.nx)
+        out = [np.empty(shape) for _ in range(4)]
+        self._chk(lib().imexlb_ensemble_get_moments(self.h, int(d), *[dptr(a) for a in out]))
+        return out
+
+    def set_launch(self, block, cpt, grid):
+        self._chk(lib().imexlb_ensemble_set_launch(self.h, int(block), int(cpt), int(grid)))
+
+    def autotune(self, cache_path=None, steps=6):
+        ch = (ctypes.c_int32 * 3)()
+        self._chk(lib().imexlb_ensemble_autotune(self.h, cache_path.encode() if cache_path else None,
+                                                 int(steps), ch))
+        return tuple(ch)
+
+    def info(self):
+        out = (ctypes.c_int64 * 7)()
+        self._chk(lib().imexlb_ensemble_info(self.h, out))
+        keys = ("cells", "tiles", "grid", "block", "cells_per_thread", "device_bytes", "steps")
+        return dict(zip(keys, list(out)))

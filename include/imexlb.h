@@ -76,6 +76,9 @@ typedef struct imexlb_config {
     int32_t cells_per_thread; /* 0 = default */
     int32_t xcd_remap;        /* -1 = default, 0 off, 1 on */
     int32_t device;           /* HIP device ordinal */
+    int32_t stride_pad;       /* bytes added between populations, multiple of
+                                 256; -1 = default.  Keeps the 27 population
+                                 streams off a common power-of-two alignment */
 } imexlb_config;
 
 typedef struct imexlb_handle imexlb_handle;
@@ -148,6 +151,49 @@ const char *imexlb_strerror(int status);
 /* Device copy of `bytes` bytes (read + write), `reps` times: measured
  * bandwidth ceiling (GB/s counted as read + write bytes / time). */
 int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch);
+
+/* Launch-shape autotuner (no counterpart in the reference).  Times every
+ * workgroup size x cells per thread x XCD tile order on this handle for
+ * `steps` steps each, keeps the fastest and appends it to the JSON-lines
+ * cache at cache_path (NULL: no cache; a cache hit skips the timing).  All
+ * shapes give bit-identical results.  The handle is re-initialised after
+ * tuning.  chosen[3] (may be NULL) receives block, cells per thread, XCD
+ * order.  Fused engine only. */
+int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t *chosen);
+
+/* ---- batched ensemble (no counterpart in the reference) ----------------
+ * Many independent domains -- TGV boxes and config-mode cavities of uneven
+ * size and Reynolds number -- advanced in lockstep by one persistent launch
+ * per step over a table of tiles, optionally replayed from a captured HIP
+ * graph.  Each member steps bit for bit as the fused engine steps it alone. */
+typedef struct imexlb_domain {
+    int32_t case_id;          /* IMEXLB_CASE_TGV or IMEXLB_CASE_CAVITY (config mode) */
+    int32_t nx, ny, nz;
+    double tau0, rho0, u0, lid_u;
+} imexlb_domain;
+typedef struct imexlb_ensemble imexlb_ensemble;
+
+#define IMEXLB_ENS_PERSISTENT 0 /* one persistent launch per step */
+#define IMEXLB_ENS_GRAPH 1      /* persistent launches replayed from a HIP graph */
+#define IMEXLB_ENS_PER_DOMAIN 2 /* one single-domain launch per domain per step */
+
+/* n domains drawn from seed: alternating TGV / cavity, edge in
+ * {32,48,64,80,96}, Re in [50,1000], lid speed in [0.05,0.1], TGV U0 0.05. */
+int imexlb_ensemble_generate(uint64_t seed, int n, imexlb_domain *out);
+int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int storage, int device,
+                           imexlb_ensemble **out);
+int imexlb_ensemble_destroy(imexlb_ensemble *e);
+int imexlb_ensemble_initialize(imexlb_ensemble *e);
+int imexlb_ensemble_step(imexlb_ensemble *e, int nsteps, int mode, int moments_every);
+int imexlb_ensemble_step_timed(imexlb_ensemble *e, int nsteps, int mode, int moments_every, float *ms);
+int imexlb_ensemble_get_moments(imexlb_ensemble *e, int domain, double *p, double *u, double *v, double *w);
+/* out[7]: cells, tiles, grid, block, cells per thread, device bytes, steps */
+int imexlb_ensemble_info(imexlb_ensemble *e, int64_t *out);
+int imexlb_ensemble_set_launch(imexlb_ensemble *e, int block, int cells_per_thread, int grid);
+/* As imexlb_autotune, over block x cells per thread x persistent grid size,
+ * timed with graph replay; chosen[3] = block, cells per thread, grid. */
+int imexlb_ensemble_autotune(imexlb_ensemble *e, const char *cache_path, int steps, int32_t *chosen);
+const char *imexlb_ensemble_last_error(const imexlb_ensemble *e);
 
 /* ---- host-side pieces of the reference, no device use ----------------- */
 

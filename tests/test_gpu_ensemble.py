@@ -1,0 +1,104 @@
+"""GPU tests of the batched ensemble and the autotuners: every ensemble member
+steps bit for bit as the single-domain fused engine steps it alone, in every
+launch mode and launch shape."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PI_R = 3.1415926
+
+
+def small_domains():
+    doms = []
+    sizes = [(16, 16, 16), (24, 20, 18), (20, 12, 16), (12, 14, 10), (18, 18, 18)]
+    for k, (nx, ny, nz) in enumerate(sizes):
+        if k % 2 == 0:
+            re = 80.0 + 40 * k
+            doms.append(dict(case_id=0, nx=nx, ny=ny, nz=nz, tau0=0.05 * nx / re / (1 / 3) / 2 / PI_R,
+                             rho0=1.0, u0=0.05, lid_u=0.0))
+        else:
+            lid = 0.06 + 0.01 * k
+            re = 100.0 + 50 * k
+            doms.append(dict(case_id=1, nx=nx, ny=ny, nz=nz, tau0=lid * nx / re / (1 / 3), rho0=1.0, u0=0.0,
+                             lid_u=lid))
+    return doms
+
+
+def single(d, dtype, storage):
+    from imexlb_amd import LBM
+    g = LBM(d["nx"], d["ny"], d["nz"], d["tau0"], rho0=d["rho0"], u0=d["u0"] if d["case_id"] == 0 else 0.0,
+            case=d["case_id"], mode="config" if d["case_id"] == 1 else "faithful", dtype=dtype,
+            storage=storage, lid_u=d["lid_u"])
+    g.Initialize()
+    return g
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+@pytest.mark.parametrize("mode", ["persistent", "graph", "per_domain"])
+def test_ensemble_members_step_as_alone(dtype, storage, mode):
+    from imexlb_amd import Ensemble
+    doms = small_domains()
+    e = Ensemble(doms, dtype=dtype, storage=storage)
+    e.Initialize()
+    singles = [single(d, dtype, storage) for d in doms]
+    for nsteps in (7, 10):
+        e.step(nsteps, mode=mode)
+        for k, g in enumerate(singles):
+            g.step(nsteps)
+            for a, b in zip(e.moments(k), g.moments()):
+                assert np.array_equal(a, b), (k, nsteps)
+
+
+def test_ensemble_launch_shapes_agree():
+    from imexlb_amd import Ensemble
+    doms = small_domains()
+    ref = None
+    for block, cpt, grid in [(256, 1, 2048), (128, 2, 64), (512, 1, 7), (64, 2, 1)]:
+        e = Ensemble(doms, dtype="f64", storage="aa")
+        e.set_launch(block, cpt, grid)
+        e.Initialize()
+        e.step(9, mode="graph")
+        m = [e.moments(k) for k in range(len(doms))]
+        if ref is None:
+            ref = m
+        else:
+            for a, b in zip(m, ref):
+                for x, y in zip(a, b):
+                    assert np.array_equal(x, y)
+
+
+def test_ensemble_autotune_and_cache(tmp_path):
+    from imexlb_amd import Ensemble
+    doms = small_domains()
+    cache = str(tmp_path / "tune.jsonl")
+    e = Ensemble(doms, dtype="f32", storage="pull")
+    e.Initialize()
+    choice = e.autotune(cache, steps=2)
+    assert choice[0] in (128, 256, 512) and choice[1] in (1, 2) and choice[2] > 0
+    assert os.path.getsize(cache) > 0
+    e2 = Ensemble(doms, dtype="f32", storage="pull")
+    e2.Initialize()
+    assert e2.autotune(cache, steps=2) == choice  # served from the cache
+    e.step(5)
+    e2.step(5)
+    for k in range(len(doms)):
+        for a, b in zip(e.moments(k), e2.moments(k)):
+            assert np.array_equal(a, b)
+
+
+def test_single_autotune_keeps_results(tmp_path):
+    from imexlb_amd import LBM
+    tau = 0.05 * 32 / 100 / (1 / 3) / 2 / PI_R
+    a = LBM(32, 24, 20, tau, storage="aa")
+    b = LBM(32, 24, 20, tau, storage="aa")
+    choice = a.autotune(str(tmp_path / "t.jsonl"), steps=2)
+    assert choice[0] in (64, 128, 256, 512)
+    b.Initialize()
+    a.step(11)
+    b.step(11)
+    for x, y in zip(a.moments(), b.moments()):
+        assert np.array_equal(x, y)
