@@ -36,10 +36,14 @@ def tgv_tau(n, re, u0):
     return u0 * n / re / (1.0 / 3.0) / 2 / PI_R
 
 
-def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True):
+def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None):
     from imexlb_amd import LBM
+    kw = {}
+    if launch:
+        kw = dict(block=launch[0], cells_per_thread=launch[1], xcd_remap=launch[2])
+        tune = False
     g = LBM(n, n, n, tgv_tau(n, re, u0), u0=u0, case="tgv", engine="fused", dtype=dtype,
-            storage=storage)
+            storage=storage, **kw)
     if tune:
         # launch shape from the in-project autotuner (re-initialises the state)
         choice = g.autotune(os.environ.get("IMEXLB_TUNE_CACHE"), steps=8)
@@ -62,12 +66,14 @@ def vel_l2(a, b):
 
 
 def traffic_from_profiles(workload):
+    """HBM bytes per launch of the workload's kernel, from the calibrated
+    FETCH_SIZE/WRITE_SIZE passes recorded under profiles/ (None if absent)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            t = json.load(f)
-        return t.get(workload)
-    except (OSError, ValueError):
+            t = json.load(f).get(workload)
+        return (t["bytes_per_launch"], t["ratio"], t["source"]) if t else None
+    except (OSError, ValueError, KeyError):
         return None
 
 
@@ -100,6 +106,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--launch", default=None,
+                    help="block,cells_per_thread,xcd_order for FP64 (skips the autotuner)")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--re", type=float, default=1600.0)
@@ -120,7 +128,8 @@ def main():
     n, K, W = a.n, a.steps, a.warmup
     cells = n ** 3
     log(f"[bench] TGV {n}^3 {a.storage} FP64: warmup {W}, timed {K}")
-    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune)
+    launch = tuple(int(v) for v in a.launch.split(",")) if a.launch else None
+    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune, launch)
     glups64 = cells * K / (ms64 * 1e-3) / 1e9
     per_launch_ms = ms64 / K
     bytes_lup = info64["bytes_per_lup"]
@@ -147,6 +156,7 @@ def main():
     gbps_copy, _ = copy_bandwidth(4 << 30, reps=10)
     log(f"[bench] device copy {gbps_copy:.0f} GB/s")
     workload = f"tgv{n}_{a.storage}_f64"
+    traffic = traffic_from_profiles(workload)
     cpu = None
     if not a.no_cpu:
         log("[bench] CPU baseline")
@@ -169,7 +179,10 @@ def main():
                                f"{a.storage.upper()} storage, FP64 populations and collision",
                    "cells": cells, "storage": a.storage, "re": a.re, "u0": a.u0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_GBS, "traffic": traffic_from_profiles(workload),
+                     "frac": achieved / PEAK_GBS, "traffic": traffic[0] if traffic else None,
+                     "traffic_over_algorithmic": traffic[1] if traffic else None,
+                     "traffic_source": traffic[2] if traffic else None,
+                     "frac_of_measured_copy": achieved / gbps_copy,
                      "bytes_per_lup": bytes_lup, "launch": {k: info64[k] for k in ("block", "cells_per_thread", "xcd_remap")},
                      "kernel": "k_fused (FP64, periodic, "
                      + ("AA" if a.storage == "aa" else "pull") + ")",

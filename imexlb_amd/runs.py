@@ -1,0 +1,221 @@
+"""Long runs of the benchmark configurations on one GPU; each prints one JSON
+line.
+
+    python -m imexlb_amd.runs precision [--n 512] [--steps 5000]
+    python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000]
+    python -m imexlb_amd.runs sphere    [--steps 50000]
+    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256]
+
+precision: TGV 512^3, U0 0.05, Re 1600, AA storage in FP64 / FP32 / FP16
+    (shifted); kinetic energy against the analytic early-time decay
+    E0 exp(-6 nu k^2 t), and the velocity L2 error of FP32/FP16 against FP64.
+cavity: 128^3, Re 400, lid 0.1, halfway bounce-back with the lid momentum
+    term; fused pull, fused AA and the phased engine compared every 1000 steps.
+sphere: 512x256x256 channel, sphere D = 40 at (128, 128, 128), inlet 0.05,
+    Re 200; FP64 / FP32 / FP16 pull; drag proxy every 1000 steps and the final
+    velocity field compared with FP64.
+ensemble: 256 domains from seed 20261202 (TGV / cavity, edges 32..96,
+    Re 50..1000, lid 0.05..0.1), FP32 shifted storage; persistent launch with
+    graph replay vs persistent launches vs one launch per domain; per-domain
+    velocity error against the same ensemble in FP64.
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import LBM, Ensemble, copy_bandwidth
+
+PI_R = 3.1415926
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def l2(a, b):
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
+    den = sum(float(np.sum(y ** 2)) for y in b)
+    return math.sqrt(num / den) if den > 0 else float("nan")
+
+
+def precision(a):
+    n, steps = a.n, a.steps
+    u0, re = 0.05, 1600.0
+    tau = u0 * n / re / (1.0 / 3.0) / 2 / PI_R
+    nu = tau / 3.0
+    k = 2 * PI_R / n
+    samples = sorted({s for s in (1, 100, 500, 1000, 2000, 3000, 4000, 5000, steps) if s <= steps})
+    runs = {}
+    for dt in ("f64", "f32", "f16"):
+        g = LBM(n, n, n, tau, u0=u0, case="tgv", dtype=dt, storage="aa")
+        g.autotune(None, steps=4)
+        runs[dt] = g
+    ke0 = None
+    out = {"workload": f"TGV {n}^3 Re {re:g} U0 {u0} AA", "nu": nu, "samples": []}
+    done = 0
+    t_dev = {dt: 0.0 for dt in runs}
+    for s in samples:
+        rec = {"step": s}
+        vel = {}
+        for dt, g in runs.items():
+            t_dev[dt] += g.step_timed(s - done, moments_every=s - done)
+            ke, mass = g.diagnostics()
+            rec[f"ke_{dt}"] = ke
+            rec[f"mass_{dt}"] = mass
+            p, u, v, w = g.moments()
+            vel[dt] = (u, v, w)
+        done = s
+        if ke0 is None:
+            ke0 = rec["ke_f64"] / math.exp(-6 * nu * k * k * s)
+        rec["ke_analytic"] = ke0 * math.exp(-6 * nu * k * k * s)
+        rec["ke_f64_over_analytic"] = rec["ke_f64"] / rec["ke_analytic"]
+        for dt in ("f32", "f16"):
+            rec[f"vel_l2_{dt}_vs_f64"] = l2(vel[dt], vel["f64"])
+        out["samples"].append(rec)
+        log(f"[precision] step {s}: KE/analytic {rec['ke_f64_over_analytic']:.6f}, "
+            f"L2 f32 {rec['vel_l2_f32_vs_f64']:.3e}, f16 {rec['vel_l2_f16_vs_f64']:.3e}")
+        del vel
+    cells = n ** 3
+    out["glups"] = {dt: cells * steps / (t * 1e-3) / 1e9 for dt, t in t_dev.items()}
+    out["note"] = "GLUPS here include the moment-writing steps at the samples"
+    return out
+
+
+def cavity(a):
+    n, steps = a.n, a.steps
+    lid, re = 0.1, 400.0
+    tau = 3.0 * lid * n / re
+    engines = {
+        "fused_pull": LBM(n, n, n, tau, u0=0.0, case="cavity", mode="config", storage="pull", lid_u=lid),
+        "fused_aa": LBM(n, n, n, tau, u0=0.0, case="cavity", mode="config", storage="aa", lid_u=lid),
+        "phased": LBM(n, n, n, tau, u0=0.0, case="cavity", mode="config", engine="phased", lid_u=lid),
+    }
+    for g in engines.values():
+        g.Initialize()
+    out = {"workload": f"cavity {n}^3 Re {re:g} lid {lid}", "tau0": tau, "checks": []}
+    t = {k: 0.0 for k in engines}
+    every = 1000
+    for s in range(every, steps + 1, every):
+        ref = None
+        rec = {"step": s}
+        for name, g in engines.items():
+            t0 = time.perf_counter()
+            g.step(every, moments_every=every)
+            g.sync()
+            t[name] += time.perf_counter() - t0
+            m = g.moments()
+            if ref is None:
+                ref = m
+                rec["umax"] = float(np.abs(m[1]).max())
+                rec["ke"], rec["mass"] = g.diagnostics()
+            else:
+                rec[f"{name}_bitexact_vs_fused_pull"] = all(np.array_equal(x, y) for x, y in zip(m, ref))
+        out["checks"].append(rec)
+        log(f"[cavity] step {s}: " + json.dumps(rec))
+    out["wall_s"] = t
+    out["all_bitexact"] = all(v for r in out["checks"] for k, v in r.items() if k.endswith("bitexact_vs_fused_pull"))
+    return out
+
+
+def sphere(a):
+    nx, ny, nz = 512, 256, 256
+    u_in, d, re = 0.05, 40.0, 200.0
+    tau = 3.0 * u_in * d / re
+    sph = (128.0, 128.0, 128.0, d / 2)
+    runs = {dt: LBM(nx, ny, nz, tau, u0=u_in, case="sphere", mode="config", dtype=dt, storage="pull",
+                    sphere=sph) for dt in ("f64", "f32", "f16")}
+    for g in runs.values():
+        g.autotune(None, steps=4)
+    out = {"workload": f"sphere channel {nx}x{ny}x{nz} D {d:g} Re {re:g} u_in {u_in}", "tau0": tau, "drag": []}
+    every = 1000
+    t_dev = {dt: 0.0 for dt in runs}
+    for s in range(every, a.steps + 1, every):
+        rec = {"step": s}
+        for dt, g in runs.items():
+            t_dev[dt] += g.step_timed(every, moments_every=0)
+            rec[f"fx_{dt}"] = float(g.sphere_force()[0])
+        for dt in ("f32", "f16"):
+            rec[f"fx_relerr_{dt}"] = abs(rec[f"fx_{dt}"] - rec["fx_f64"]) / abs(rec["fx_f64"])
+        out["drag"].append(rec)
+        if s % 10000 == 0 or s == every:
+            log(f"[sphere] step {s}: " + json.dumps(rec))
+    vel = {}
+    for dt, g in runs.items():
+        g.step(1, moments_every=1)
+        p, u, v, w = g.moments()
+        vel[dt] = (u, v, w)
+    out["vel_l2_vs_f64"] = {dt: l2(vel[dt], vel["f64"]) for dt in ("f32", "f16")}
+    cells = nx * ny * nz
+    out["glups"] = {dt: cells * (a.steps // every * every) / (t * 1e-3) / 1e9 for dt, t in t_dev.items()}
+    out["cd_proxy_f64"] = out["drag"][-1]["fx_f64"] / (0.5 * 1.0 * u_in ** 2 * math.pi * (d / 2) ** 2)
+    return out
+
+
+def ensemble(a):
+    doms = Ensemble.generate(20261202, a.domains)
+    cells = sum(d.nx * d.ny * d.nz for d in doms)
+    out = {"workload": f"{a.domains} domains seed 20261202, {cells} cells", "cells": cells}
+    e = Ensemble(doms, dtype="f32", storage="pull")
+    e.Initialize()
+    out["autotune"] = e.autotune(None, steps=4)
+    res = {}
+    for mode in ("graph", "persistent", "per_domain"):
+        e.Initialize()
+        e.step(2, mode=mode, moments_every=0)
+        ms = e.step_timed(a.steps, mode=mode, moments_every=0)
+        res[mode] = {"glups": cells * a.steps / (ms * 1e-3) / 1e9, "ms_per_step": ms / a.steps}
+        log(f"[ensemble] {mode}: {res[mode]['glups']:.2f} GLUPS")
+    out["modes"] = res
+    out["graph_vs_per_domain"] = res["graph"]["glups"] / res["per_domain"]["glups"]
+    # per-domain error: same ensemble in FP64, same number of steps
+    e.Initialize()
+    e.step(a.steps, mode="graph")
+    ref = Ensemble(doms, dtype="f64", storage="pull")
+    ref.Initialize()
+    ref.step(a.steps, mode="graph")
+    errs, fin64, fin32 = [], [], []
+    for k in range(len(doms)):
+        m32 = e.moments(k)
+        m64 = ref.moments(k)
+        fin64.append(all(np.isfinite(x).all() for x in m64))
+        fin32.append(all(np.isfinite(x).all() for x in m32))
+        errs.append(l2(m32[1:], m64[1:]) if fin64[-1] and fin32[-1] else float("nan"))
+    errs, fin64, fin32 = np.array(errs), np.array(fin64), np.array(fin32)
+    ok = fin64 & fin32
+    # BGK is unstable at the smallest edges and highest Reynolds numbers of
+    # the draw (tau0 -> 0); such domains blow up in FP64 as in FP32
+    out["unstable_domains_f64"] = [int(k) for k in np.where(~fin64)[0]]
+    out["unstable_domains_f32"] = [int(k) for k in np.where(~fin32)[0]]
+    out["unstable_tau0"] = [doms[k].tau0 for k in np.where(~fin64)[0]]
+    if ok.any():
+        out["per_domain_vel_l2_vs_f64"] = {
+            "domains": int(ok.sum()), "max": float(errs[ok].max()), "median": float(np.median(errs[ok])),
+            "tgv_max": float(np.nanmax(np.where(ok[0::2], errs[0::2], np.nan))),
+            "cavity_max": float(np.nanmax(np.where(ok[1::2], errs[1::2], np.nan)))}
+    out["info"] = e.info()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload", choices=["precision", "cavity", "sphere", "ensemble"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--domains", type=int, default=256)
+    a = ap.parse_args(argv)
+    defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000)}
+    dn, ds = defaults[a.workload]
+    a.n = a.n or dn
+    a.steps = a.steps or ds
+    fn = {"precision": precision, "cavity": cavity, "sphere": sphere, "ensemble": ensemble}[a.workload]
+    out = fn(a)
+    out["copy_GBps"] = copy_bandwidth(1 << 30, reps=5)[0]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
