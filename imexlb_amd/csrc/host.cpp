@@ -1,6 +1,7 @@
 // Host-only pieces of the reference behind the C-ABI: the input.in parser
 // (System) and the rank grid (CommHelper + the LBM constructor's extents).
 // Neither touches the GPU.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -183,6 +184,87 @@ int imexlb_comm_layout(int nranks, int me, int periodic, int glx, int gly, int g
         o->hi[d] = hi[d] - 1;
     }
     return IMEXLB_OK;
+}
+
+// Tecplot binary (#!TDV112) file of one ordered zone with the variables
+// x y z u v w p, the layout LBM::MPIoutput writes (tgv/src/lbm.cpp:335-669,
+// lid_driven_cavity/src/lbm.cpp:351-674): a 332-byte header -- title, seven
+// variable names, zone "ZONE 001" with the solution time and I/J/K, all
+// variables FP64 -- then the 14 min/max doubles and the seven arrays, x
+// fastest.  Coordinates are x / (nx - 1) etc. on [0, 1]; min/max of u v w p
+// are taken over the given arrays.
+int imexlb_tecplot_write(const char *path, double solution_time, int nx, int ny, int nz, const double *p,
+                         const double *u, const double *v, const double *w)
+{
+    if (!path || nx < 1 || ny < 1 || nz < 1 || !p || !u || !v || !w) return IMEXLB_EINVAL;
+    const size_t n = (size_t)nx * ny * nz;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return IMEXLB_EIO;
+    bool ok = true;
+    auto I = [&](int32_t v_) { ok = ok && std::fwrite(&v_, 4, 1, fp) == 1; };
+    auto F = [&](float v_) { ok = ok && std::fwrite(&v_, 4, 1, fp) == 1; };
+    auto D = [&](double v_) { ok = ok && std::fwrite(&v_, 8, 1, fp) == 1; };
+    ok = std::fwrite("#!TDV112", 1, 8, fp) == 8;
+    I(1);  // byte order
+    I(0);  // file type: full
+    I(0);  // empty title
+    I(7);
+    for (int c : {'x', 'y', 'z', 'u', 'v', 'w', 'p'}) {
+        I(c);
+        I(0);
+    }
+    F(299.0f);  // zone marker
+    for (int c : {'Z', 'O', 'N', 'E', ' ', '0', '0', '1'}) I(c);
+    I(0);
+    I(-1);              // parent zone
+    D(solution_time);   // solution time
+    I(0);               // strand (static)
+    I(-1);              // zone colour
+    I(0);               // ordered zone
+    I(0);               // no variable location list
+    I(0);               // no raw face neighbours
+    I(0);               // no user face connections
+    I(nx);
+    I(ny);
+    I(nz);
+    I(0);               // no auxiliary data
+    F(357.0f);          // end of header
+    F(299.0f);          // data section
+    for (int k = 0; k < 7; ++k) I(2);  // FP64
+    I(0);               // no passive variables
+    I(0);               // no shared variables
+    I(-1);              // no shared connectivity
+    const double *fields[4] = {u, v, w, p};
+    double mn[4], mx[4];
+    for (int k = 0; k < 4; ++k) {
+        mn[k] = *std::min_element(fields[k], fields[k] + n);
+        mx[k] = *std::max_element(fields[k], fields[k] + n);
+    }
+    for (int k = 0; k < 3; ++k) {
+        D(0.0);
+        D(1.0);
+    }
+    for (int k = 0; k < 4; ++k) {
+        D(mn[k]);
+        D(mx[k]);
+    }
+    // coordinates, then the fields, x fastest
+    std::vector<double> row((size_t)nx);
+    const int ext[3] = {nx, ny, nz};
+    for (int axis = 0; axis < 3 && ok; ++axis) {
+        const double den = ext[axis] > 1 ? (double)(ext[axis] - 1) : 1.0;
+        for (int z = 0; z < nz && ok; ++z)
+            for (int y = 0; y < ny && ok; ++y) {
+                for (int x = 0; x < nx; ++x) {
+                    const int c = axis == 0 ? x : (axis == 1 ? y : z);
+                    row[x] = (double)c / den;
+                }
+                ok = std::fwrite(row.data(), 8, (size_t)nx, fp) == (size_t)nx;
+            }
+    }
+    for (int k = 0; k < 4 && ok; ++k) ok = std::fwrite(fields[k], 8, n, fp) == n;
+    if (std::fclose(fp) != 0) ok = false;
+    return ok ? IMEXLB_OK : IMEXLB_EIO;
 }
 
 }  // extern "C"

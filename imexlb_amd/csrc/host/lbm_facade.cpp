@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 #include <iostream>
+#include <vector>
 
 namespace imexlb {
 
@@ -104,6 +105,17 @@ void LBM::exchange() { check(imexlb_exchange(h_)); }
 void LBM::unpack() { check(imexlb_unpack(h_)); }
 void LBM::Streaming() { check(imexlb_stream(h_)); }
 void LBM::Update() { check(imexlb_update(h_)); }
+void LBM::MPIoutput(int n)
+{
+    const size_t cells = (size_t)comm.l_l[0] * comm.l_l[1] * comm.l_l[2];
+    std::vector<double> p(cells), u(cells), v(cells), w(cells);
+    check(imexlb_get_moments(h_, p.data(), u.data(), v.data(), w.data()));
+    const std::string path = "output" + std::to_string(n) + ".plt";
+    check(imexlb_tecplot_write(path.c_str(), (double)n, comm.l_l[0], comm.l_l[1], comm.l_l[2], p.data(), u.data(),
+                               v.data(), w.data()));
+    if (comm.me == 0) std::printf("\nThe result %d is writen\n\n============================\n", n);
+}
+
 void LBM::step(int n, int every) { check(imexlb_step(h_, n, every)); }
 float LBM::step_timed(int n, int every)
 {

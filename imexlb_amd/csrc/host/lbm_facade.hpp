@@ -65,6 +65,7 @@ struct LBM {
     void unpack();
     void Streaming();
     void Update();
+    void MPIoutput(int n);  // output<n>.plt, tgv/src/lbm.cpp:335-669
 
     // beyond the reference: fused fast path, data access, diagnostics
     void step(int nsteps, int moments_every);

@@ -9,6 +9,8 @@
 //          [--engine fused|phased] [--dtype f64|f32|f16] [--storage pull|aa]
 //          [--input input.in] [--steps N] [--honour-input] [--fast]
 //          [--lid U] [--sphere cx,cy,cz,r] [--tune CACHE] [--device D]
+//          [--output]   (Tecplot output<n>.plt every inter steps, as the
+//                        cavity driver does: lid_driven_cavity/main.cpp:26,39-44)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -38,7 +40,7 @@ int main(int argc, char **argv)
     Options opt;
     std::string input = "input.in";
     int steps = -1;
-    bool honour_input = false, fast = false, have_sphere = false;
+    bool honour_input = false, fast = false, have_sphere = false, output = false;
     std::string tune;
     for (int a = 1; a < argc; ++a) {
         std::string k = argv[a];
@@ -58,6 +60,7 @@ int main(int argc, char **argv)
         else if (k == "--steps") steps = std::atoi(val().c_str());
         else if (k == "--honour-input") honour_input = true;
         else if (k == "--fast") fast = true;
+        else if (k == "--output") output = true;
         else if (k == "--lid") opt.lid_u = std::atof(val().c_str());
         else if (k == "--device") opt.device = std::atoi(val().c_str());
         else if (k == "--tune") tune = val();
@@ -153,6 +156,7 @@ int main(int argc, char **argv)
                     double ke = 0, mass = 0;
                     l1.diagnostics(ke, mass);
                     std::printf("time-step = %d  KE = %.12e  mass = %.12e\n", it, ke, mass);
+                    if (output) l1.MPIoutput(it / s1.inter);
                 }
             }
             end = now_s();

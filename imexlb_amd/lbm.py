@@ -140,6 +140,15 @@ class LBM:
         self._chk(lib().imexlb_sphere_force(self.h, dptr(out)))
         return out
 
+    def MPIoutput(self, n, directory="."):
+        """Write output<n>.plt (Tecplot, x y z u v w p) as the reference's
+        LBM::MPIoutput does (tgv/src/lbm.cpp:335-669)."""
+        import os
+        p, u, v, w = self.moments()
+        path = os.path.join(directory, f"output{int(n)}.plt")
+        tecplot_write(path, float(n), p, u, v, w)
+        return path
+
     def autotune(self, cache_path=None, steps=10):
         """Pick the fastest launch shape (results are shape-independent); the
         handle is re-initialised afterwards.  Returns (block, cpt, xcd)."""
@@ -171,6 +180,14 @@ def comm_layout(nranks, me, periodic, glx, gly, glz):
     return {"nranks": c.nranks, "me": c.me, "rx": c.rx, "ry": c.ry, "rz": c.rz, "px": c.px, "py": c.py,
             "pz": c.pz, "neighbours": list(c.neighbours), "l_l": list(c.l_l), "lo": list(c.lo),
             "hi": list(c.hi)}
+
+
+def tecplot_write(path, solution_time, p, u, v, w):
+    """Tecplot #!TDV112 file of x y z u v w p; fields shaped [z][y][x]."""
+    arrs = [as_f64(a) for a in (p, u, v, w)]
+    nz, ny, nx = arrs[0].shape
+    check(lib().imexlb_tecplot_write(str(path).encode(), float(solution_time), nx, ny, nz,
+                                     *[dptr(a) for a in arrs]))
 
 
 def copy_bandwidth(nbytes, reps=20, device=0):

@@ -224,6 +224,13 @@ typedef struct imexlb_comm {
 } imexlb_comm;
 int imexlb_comm_layout(int nranks, int me, int periodic, int glx, int gly, int glz, imexlb_comm *out);
 
+/* Tecplot binary (#!TDV112) field file: variables x y z u v w p of one
+ * ordered nx*ny*nz zone, FP64, x fastest -- the file LBM::MPIoutput writes
+ * (tgv/src/lbm.cpp:335-669; lid_driven_cavity/src/lbm.cpp:351-674), here for
+ * one rank from caller-owned host arrays (as returned by imexlb_get_moments). */
+int imexlb_tecplot_write(const char *path, double solution_time, int nx, int ny, int nz, const double *p,
+                         const double *u, const double *v, const double *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,3 +110,37 @@ def test_strerror_covers_every_status():
     L = im.lib()
     for s in range(0, -8, -1):
         assert L.imexlb_strerror(s).decode() not in ("", "unknown status")
+
+
+def test_tecplot_writer_layout(tmp_path):
+    """imexlb_tecplot_write produces the reference's #!TDV112 file
+    (tgv/src/lbm.cpp:421-665): 332-byte header, then x y z u v w p."""
+    from imexlb_amd import tecplot_write
+    rng = np.random.default_rng(5)
+    nz, ny, nx = 3, 4, 5
+    p, u, v, w = (rng.standard_normal((nz, ny, nx)) for _ in range(4))
+    path = tmp_path / "output7.plt"
+    tecplot_write(path, 7.0, p, u, v, w)
+    raw = path.read_bytes()
+    assert len(raw) == 332 + 7 * nx * ny * nz * 8
+    assert raw[:8] == b"#!TDV112"
+    ints = np.frombuffer(raw[8:80], dtype="<i4")
+    assert ints[:4].tolist() == [1, 0, 0, 7]
+    assert ints[4:18].tolist() == [120, 0, 121, 0, 122, 0, 117, 0, 118, 0, 119, 0, 112, 0]
+    assert np.frombuffer(raw[80:84], "<f4")[0] == 299.0
+    assert np.frombuffer(raw[84:120], "<i4").tolist() == [90, 79, 78, 69, 32, 48, 48, 49, 0]
+    assert np.frombuffer(raw[120:124], "<i4")[0] == -1
+    assert np.frombuffer(raw[124:132], "<f8")[0] == 7.0
+    assert np.frombuffer(raw[132:172], "<i4").tolist() == [0, -1, 0, 0, 0, 0, nx, ny, nz, 0]
+    assert np.frombuffer(raw[172:180], "<f4").tolist() == [357.0, 299.0]
+    assert np.frombuffer(raw[180:220], "<i4").tolist() == [2] * 7 + [0, 0, -1]
+    mm = np.frombuffer(raw[220:332], "<f8")
+    assert mm[:6].tolist() == [0, 1, 0, 1, 0, 1]
+    assert mm[6:].tolist() == [u.min(), u.max(), v.min(), v.max(), w.min(), w.max(), p.min(), p.max()]
+    data = np.frombuffer(raw[332:], "<f8").reshape(7, nz, ny, nx)
+    X, Y, Z = np.meshgrid(np.arange(nx) / (nx - 1), np.arange(ny) / (ny - 1), np.arange(nz) / (nz - 1), indexing="ij")
+    assert np.array_equal(data[0], X.transpose(2, 1, 0))
+    assert np.array_equal(data[1], Y.transpose(2, 1, 0))
+    assert np.array_equal(data[2], Z.transpose(2, 1, 0))
+    for k, a in enumerate((u, v, w, p)):
+        assert np.array_equal(data[3 + k], a)
