@@ -40,14 +40,16 @@ def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None):
     from imexlb_amd import LBM
     kw = {}
     if launch:
-        kw = dict(block=launch[0], cells_per_thread=launch[1], xcd_remap=launch[2])
+        kw = dict(block=launch[0], cells_per_thread=launch[1], xcd_remap=launch[2],
+                  nt_loads=launch[3] if len(launch) > 3 else -1)
         tune = False
     g = LBM(n, n, n, tgv_tau(n, re, u0), u0=u0, case="tgv", engine="fused", dtype=dtype,
             storage=storage, **kw)
     if tune:
         # launch shape from the in-project autotuner (re-initialises the state)
         choice = g.autotune(os.environ.get("IMEXLB_TUNE_CACHE"), steps=8)
-        log(f"[bench] {dtype}: autotuned block {choice[0]}, cells/thread {choice[1]}, xcd order {choice[2]}")
+        log(f"[bench] {dtype}: autotuned block {choice[0]}, cells/thread {choice[1]}, xcd order {choice[2]}, "
+            f"nt loads {choice[3]}")
     else:
         g.Initialize()
     g.step(warmup, moments_every=0)
@@ -107,7 +109,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--launch", default=None,
-                    help="block,cells_per_thread,xcd_order for FP64 (skips the autotuner)")
+                    help="block,cells_per_thread,xcd_order,nt_loads for FP64 (skips the autotuner)")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--re", type=float, default=1600.0)
@@ -183,7 +185,7 @@ def main():
                      "traffic_over_algorithmic": traffic[1] if traffic else None,
                      "traffic_source": traffic[2] if traffic else None,
                      "frac_of_measured_copy": achieved / gbps_copy,
-                     "bytes_per_lup": bytes_lup, "launch": {k: info64[k] for k in ("block", "cells_per_thread", "xcd_remap")},
+                     "bytes_per_lup": bytes_lup, "launch": {k: info64[k] for k in ("block", "cells_per_thread", "xcd_remap", "nt_loads")},
                      "kernel": "k_fused (FP64, periodic, "
                      + ("AA" if a.storage == "aa" else "pull") + ")",
                      "measured_copy_GBps": gbps_copy},

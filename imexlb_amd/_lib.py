@@ -33,7 +33,7 @@ class Config(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
         ("xcd_remap", ctypes.c_int32), ("device", ctypes.c_int32),
-        ("stride_pad", ctypes.c_int32),
+        ("stride_pad", ctypes.c_int32), ("nt_loads", ctypes.c_int32),
     ]
 
 
@@ -41,7 +41,7 @@ class Info(ctypes.Structure):
     _fields_ = [("cells", ctypes.c_int64), ("steps", ctypes.c_int64),
                 ("bytes_per_lup", ctypes.c_double), ("device_bytes", ctypes.c_int64),
                 ("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
-                ("xcd_remap", ctypes.c_int32)]
+                ("xcd_remap", ctypes.c_int32), ("nt_loads", ctypes.c_int32)]
 
 
 class SystemC(ctypes.Structure):
@@ -97,6 +97,7 @@ SIGNATURES = [
     ("imexlb_comm_layout", _i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(Comm)]),
     ("imexlb_tecplot_write", _i, [ctypes.c_char_p, ctypes.c_double, _i, _i, _i, _dp, _dp, _dp, _dp]),
     ("imexlb_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
+    ("imexlb_set_launch", _i, [_vp, _i, _i, _i, _i]),
     ("imexlb_ensemble_generate", _i, [ctypes.c_uint64, _i, ctypes.POINTER(EnsembleDomain)]),
     ("imexlb_ensemble_create", _i, [ctypes.POINTER(EnsembleDomain), _i, _i, _i, _i, ctypes.POINTER(_vp)]),
     ("imexlb_ensemble_destroy", _i, [_vp]),

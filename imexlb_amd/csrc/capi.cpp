@@ -41,7 +41,7 @@ struct imexlb_handle {
     double corr[Q] = {};
     double feq_in[Q] = {};
     double *tgv_tab = nullptr;
-    int block = 256, cpt = 1, xcd = 1;
+    int block = 256, cpt = 1, xcd = 1, ntl = 0;
     int64_t steps = 0;
     int64_t mom_step = -1;   // step whose moments are in mom[]
     bool initialized = false;
@@ -91,6 +91,7 @@ int imexlb_config_init(imexlb_config *c)
     c->seed = 5374857ULL;
     c->xcd_remap = -1;
     c->stride_pad = -1;
+    c->nt_loads = -1;
     return IMEXLB_OK;
 }
 
@@ -178,6 +179,7 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.nblocks = (h->ncell + tile - 1) / tile;
     P.xcd_remap = h->xcd;
     P.cpt = h->cpt;
+    P.ntl = h->ntl;
     P.divx.init(c.nx);
     P.divy.init(c.ny);
 }
@@ -283,6 +285,7 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
     h->block = c.block ? c.block : 256;
     h->cpt = c.cells_per_thread ? c.cells_per_thread : 1;
     h->xcd = c.xcd_remap < 0 ? 1 : c.xcd_remap;
+    h->ntl = c.nt_loads < 0 ? 1 : (c.nt_loads ? 1 : 0);
     // stored shift S: f = g + w S for reduced precision; FP64 stores f itself
     if (c.dtype != IMEXLB_F64) h->shift = c.rho0;
     const size_t lat_bytes = (size_t)Q * h->pstride * h->elem;
@@ -671,6 +674,7 @@ int imexlb_get_info(imexlb_handle *h, imexlb_info *info)
     info->block = h->block;
     info->cells_per_thread = h->cpt;
     info->xcd_remap = h->xcd;
+    info->nt_loads = h->ntl;
     return IMEXLB_OK;
 }
 
@@ -693,26 +697,44 @@ int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t
         const int blocks[4] = {64, 128, 256, 512};
         for (int b : blocks)
             for (int cpt = 1; cpt <= 2; ++cpt)
-                for (int xcd = 0; xcd <= 1; ++xcd) {
-                    h->block = b;
-                    h->cpt = cpt;
-                    h->xcd = xcd;
-                    if ((rc = imexlb_step(h, 2, 0))) return rc;
-                    float ms = 0;
-                    if ((rc = imexlb_step_timed(h, steps, 0, &ms))) return rc;
-                    if (ms < best.ms) best = TuneChoice{b, cpt, xcd, ms};
-                }
+                for (int xcd = 0; xcd <= 1; ++xcd)
+                    for (int ntl = 0; ntl <= 1; ++ntl) {
+                        h->block = b;
+                        h->cpt = cpt;
+                        h->xcd = xcd;
+                        h->ntl = ntl;
+                        if ((rc = imexlb_step(h, 2, 0))) return rc;
+                        float ms = 0;
+                        if ((rc = imexlb_step_timed(h, steps, 0, &ms))) return rc;
+                        if (ms < best.ms) best = TuneChoice{b, cpt, xcd, ntl, ms};
+                    }
         tune_cache_write(cache_path, key, best);
     }
     h->block = best.block;
     h->cpt = best.cpt;
     h->xcd = best.grid_or_xcd;
+    h->ntl = best.ntl;
     if (chosen) {
         chosen[0] = best.block;
         chosen[1] = best.cpt;
         chosen[2] = best.grid_or_xcd;
+        chosen[3] = best.ntl;
     }
     return imexlb_initialize(h);
+}
+
+int imexlb_set_launch(imexlb_handle *h, int block, int cpt, int xcd, int ntl)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (h->ph) return fail(h, IMEXLB_EUNSUPPORTED, "the phased engine has no launch shapes");
+    if (block < 64 || block > 512 || block % 64 || (cpt != 1 && cpt != 2) || (xcd != 0 && xcd != 1) ||
+        (ntl != 0 && ntl != 1))
+        return fail(h, IMEXLB_EINVAL, "launch shape %d/%d/%d/%d", block, cpt, xcd, ntl);
+    h->block = block;
+    h->cpt = cpt;
+    h->xcd = xcd;
+    h->ntl = ntl;
+    return IMEXLB_OK;
 }
 
 int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch)

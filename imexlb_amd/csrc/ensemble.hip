@@ -34,7 +34,7 @@ struct EnsParams {
     int write_moments;
 };
 
-template <typename S, int BC, int PAT, int CPT>
+template <typename S, int BC, int PAT, int CPT, bool NTL>
 __device__ __forceinline__ void tile_run(const FusedParams &P, uint32_t first, bool wm)
 {
     using R = typename RealOf<S>::type;
@@ -43,7 +43,7 @@ __device__ __forceinline__ void tile_run(const FusedParams &P, uint32_t first, b
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         cell_setup<BC>(P, first + (uint32_t)r * blockDim.x + threadIdx.x, k[r]);
-        if (k[r].live) cell_gather<S, BC, PAT>(P, k[r], f[r]);
+        if (k[r].live) cell_gather<S, BC, PAT, NTL>(P, k[r], f[r]);
     }
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
@@ -61,8 +61,10 @@ __global__ void __launch_bounds__(512) k_ensemble(const EnsParams EP)
     for (uint32_t t = blockIdx.x; t < EP.ntiles; t += gridDim.x) {
         const uint2 tl = EP.tiles[t];
         const FusedParams &P = EP.doms[2 * tl.x + (uint32_t)EP.parity];
-        if (P.bc == BC_PERIODIC) tile_run<S, BC_PERIODIC, PAT, CPT>(P, tl.y, wm);
-        else tile_run<S, BC_CAVITY, PAT, CPT>(P, tl.y, wm);
+        // non-temporal loads of unshared lines (measured faster on every
+        // storage type of the single-domain kernel)
+        if (P.bc == BC_PERIODIC) tile_run<S, BC_PERIODIC, PAT, CPT, true>(P, tl.y, wm);
+        else tile_run<S, BC_CAVITY, PAT, CPT, true>(P, tl.y, wm);
     }
 }
 
@@ -538,8 +540,8 @@ bool tune_cache_read(const char *path, const std::string &key, TuneChoice &out)
         if (line.find(pat) == std::string::npos) continue;
         TuneChoice c{};
         const char *s = line.c_str() + line.find(pat) + pat.size();
-        if (sscanf(s, ", \"block\": %d, \"cpt\": %d, \"third\": %d, \"ms\": %f", &c.block, &c.cpt, &c.grid_or_xcd,
-                   &c.ms) == 4) {
+        if (sscanf(s, ", \"block\": %d, \"cpt\": %d, \"third\": %d, \"ntl\": %d, \"ms\": %f", &c.block, &c.cpt,
+                   &c.grid_or_xcd, &c.ntl, &c.ms) == 5) {
             out = c;
             found = true;  // last entry for the key wins
         }
@@ -552,7 +554,7 @@ void tune_cache_write(const char *path, const std::string &key, const TuneChoice
     if (!path || !*path) return;
     std::ofstream o(path, std::ios::app);
     o << "{\"key\": \"" << key << "\", \"block\": " << c.block << ", \"cpt\": " << c.cpt << ", \"third\": "
-      << c.grid_or_xcd << ", \"ms\": " << c.ms << "}\n";
+      << c.grid_or_xcd << ", \"ntl\": " << c.ntl << ", \"ms\": " << c.ms << "}\n";
 }
 
 }  // namespace imx
@@ -586,7 +588,7 @@ int imexlb_ensemble_autotune(imexlb_ensemble *e, const char *cache_path, int ste
                     if ((rc = imexlb_ensemble_step(e, 2, 1, 0))) return rc;  // warm: capture + one replay
                     float ms = 0;
                     if ((rc = imexlb_ensemble_step_timed(e, steps, 1, 0, &ms))) return rc;
-                    if (ms < best.ms) best = TuneChoice{b, c, g, ms};
+                    if (ms < best.ms) best = TuneChoice{b, c, g, 0, ms};
                 }
         tune_cache_write(cache_path, key, best);
     }

@@ -25,15 +25,23 @@ namespace imx {
 template <typename S> struct RealOf { using type = float; };
 template <> struct RealOf<double> { using type = double; };
 
-template <typename S>
+template <typename S, bool NT = false>
 __device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_t pstride, int i, uint32_t c)
 {
     const S *p = lat + (uint64_t)i * pstride + c;
+    S x;
+    if constexpr (NT) x = __builtin_nontemporal_load(p);
+    else x = *p;
     if constexpr (std::is_same<S, _Float16>::value)
-        return (float)(*p) * H_INV_SCALE[WCLASS[i]];
+        return (float)x * H_INV_SCALE[WCLASS[i]];
     else
-        return *p;
+        return x;
 }
+
+// NTL (non-temporal loads) variant of the gather: lines that no other
+// wavefront reads -- every line of the AA even (in-place) step, the x-aligned
+// lines of the pull and AA odd steps -- are loaded without being kept in the
+// caches.  Chosen per device by the autotuner; results are identical.
 
 // Store policy: 0 non-temporal everywhere, 1 plain everywhere, 2 (default)
 // non-temporal for aligned writes and plain for the AA odd step's x-shifted
@@ -237,7 +245,7 @@ __device__ __forceinline__ uint32_t src_off(const CellCtx &k, int i)
     return k.rowb[1 - EZ[i]][1 - EY[i]] + k.xv[1 - EX[i]];
 }
 
-template <typename S, int BC, int PAT>
+template <typename S, int BC, int PAT, bool NTL = false>
 __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx &k,
                                             typename RealOf<S>::type (&f)[Q])
 {
@@ -249,7 +257,7 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
     if constexpr (PAT == PAT_AA_EVEN) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            f[i] = ld_pop<S>(src, ps, i, c);
+            f[i] = ld_pop<S, NTL>(src, ps, i, c);
             if constexpr (BC == BC_CAVITY) {
                 // the odd step stored the bounced population raw; the lid
                 // term is added on arrival, as in the pull form
@@ -263,10 +271,12 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             if constexpr (BC == BC_PERIODIC) {
-                f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
+                if (NTL && EX[i] == 0) f[i] = ld_pop<S, true>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
+                else f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
             } else if constexpr (BC == BC_CAVITY) {
                 if (!src_out(k, i)) {
-                    f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
+                    if (NTL && EX[i] == 0) f[i] = ld_pop<S, true>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
+                    else f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
                 } else {
                     // halfway bounce-back: f_i(x) = f*_bb(i)(x) (+ lid term)
                     R v = ld_pop<S>(src, ps, PAT == PAT_PULL ? BB[i] : i, c);
@@ -354,7 +364,7 @@ __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx
 
 // CPT cells per thread, blockDim.x apart; the loads of all CPT cells are
 // issued before any of them is collided, to keep more bytes in flight.
-template <typename S, int BC, int PAT, int CPT>
+template <typename S, int BC, int PAT, int CPT, bool NTL>
 __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
 {
     using R = typename RealOf<S>::type;
@@ -366,7 +376,7 @@ __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         cell_setup<BC>(P, base + (uint32_t)r * blockDim.x, k[r]);
-        if (k[r].live) cell_gather<S, BC, PAT>(P, k[r], f[r]);
+        if (k[r].live) cell_gather<S, BC, PAT, NTL>(P, k[r], f[r]);
     }
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
@@ -376,11 +386,11 @@ __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
     }
 }
 
-template <typename S, int CPT>
+template <typename S, int CPT, bool NTL>
 hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
     dim3 grid(P.nblocks), blk(block);
-#define IMX_L(B, PT) hipLaunchKernelGGL((k_fused<S, B, PT, CPT>), grid, blk, 0, st, P)
+#define IMX_L(B, PT) hipLaunchKernelGGL((k_fused<S, B, PT, CPT, NTL>), grid, blk, 0, st, P)
     if (bc == BC_PERIODIC) {
         if (pattern == PAT_PULL) IMX_L(BC_PERIODIC, PAT_PULL);
         else if (pattern == PAT_AA_EVEN) IMX_L(BC_PERIODIC, PAT_AA_EVEN);
@@ -400,8 +410,12 @@ hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block
 template <typename S>
 hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
-    if (P.cpt == 2) return launch_fused_cpt<S, 2>(bc, pattern, P, block, st);
-    return launch_fused_cpt<S, 1>(bc, pattern, P, block, st);
+    if (P.ntl) {
+        if (P.cpt == 2) return launch_fused_cpt<S, 2, true>(bc, pattern, P, block, st);
+        return launch_fused_cpt<S, 1, true>(bc, pattern, P, block, st);
+    }
+    if (P.cpt == 2) return launch_fused_cpt<S, 2, false>(bc, pattern, P, block, st);
+    return launch_fused_cpt<S, 1, false>(bc, pattern, P, block, st);
 }
 
 }  // namespace imx

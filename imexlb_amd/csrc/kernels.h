@@ -53,6 +53,7 @@ struct FusedParams {
     uint32_t nblocks;        // grid size (for the XCD-aware block order)
     int xcd_remap;           // 1: give each XCD one contiguous run of tiles
     int cpt;                 // cells per thread (1 or 2), blockDim apart
+    int ntl;                 // 1: non-temporal loads of unshared lines
     int bc;                  // boundary kind (read by the ensemble kernel)
     FastDiv divx, divy;
 };
@@ -109,7 +110,7 @@ namespace imx {
 
 // Autotuner cache (JSON lines), shared by the single-domain and ensemble tuners.
 struct TuneChoice {
-    int block, cpt, grid_or_xcd;
+    int block, cpt, grid_or_xcd, ntl;
     float ms;
 };
 std::string tune_key(int device, const char *kind, int dtype, int storage, const std::string &shape);

@@ -36,7 +36,8 @@ class LBM:
 
     def __init__(self, sx, sy, sz, tau, rho0=1.0, u0=0.05, case="tgv", mode="faithful",
                  engine="fused", dtype="f64", storage="pull", lid_u=0.1, sphere=None,
-                 seed=5374857, block=0, cells_per_thread=0, xcd_remap=-1, device=0, stride_pad=-1):
+                 seed=5374857, block=0, cells_per_thread=0, xcd_remap=-1, device=0, stride_pad=-1,
+                 nt_loads=-1):
         c = L.Config()
         check(lib().imexlb_config_init(ctypes.byref(c)))
         c.case_id = _code(CASES, case)
@@ -52,6 +53,7 @@ class LBM:
         c.block, c.cells_per_thread, c.xcd_remap, c.device = int(block), int(cells_per_thread), \
             int(xcd_remap), int(device)
         c.stride_pad = int(stride_pad)
+        c.nt_loads = int(nt_loads)
         self.cfg = c
         self.shape = (c.nz, c.ny, c.nx)
         h = ctypes.c_void_p()
@@ -151,10 +153,13 @@ class LBM:
 
     def autotune(self, cache_path=None, steps=10):
         """Pick the fastest launch shape (results are shape-independent); the
-        handle is re-initialised afterwards.  Returns (block, cpt, xcd)."""
-        ch = (ctypes.c_int32 * 3)()
+        handle is re-initialised afterwards.  Returns (block, cpt, xcd, ntl)."""
+        ch = (ctypes.c_int32 * 4)()
         self._chk(lib().imexlb_autotune(self.h, cache_path.encode() if cache_path else None, int(steps), ch))
         return tuple(ch)
+
+    def set_launch(self, block, cells_per_thread=1, xcd_order=1, nt_loads=0):
+        self._chk(lib().imexlb_set_launch(self.h, int(block), int(cells_per_thread), int(xcd_order), int(nt_loads)))
 
     def info(self):
         i = L.Info()

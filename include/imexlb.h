@@ -79,6 +79,8 @@ typedef struct imexlb_config {
     int32_t stride_pad;       /* bytes added between populations, multiple of
                                  256; -1 = default.  Keeps the 27 population
                                  streams off a common power-of-two alignment */
+    int32_t nt_loads;         /* -1 = default, 0/1: non-temporal loads of the
+                                 lines no other wavefront reads */
 } imexlb_config;
 
 typedef struct imexlb_handle imexlb_handle;
@@ -88,7 +90,7 @@ typedef struct imexlb_info {
     int64_t steps;          /* steps taken since Initialize */
     double bytes_per_lup;   /* algorithmic HBM bytes per cell update */
     int64_t device_bytes;   /* device memory held by the handle */
-    int32_t block, cells_per_thread, xcd_remap;
+    int32_t block, cells_per_thread, xcd_remap, nt_loads;
 } imexlb_info;
 
 /* Defaults: TGV, faithful, fused, FP64, pull, 64^3, tau0 = 0.0153, rho0 = 1,
@@ -153,13 +155,16 @@ const char *imexlb_strerror(int status);
 int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch);
 
 /* Launch-shape autotuner (no counterpart in the reference).  Times every
- * workgroup size x cells per thread x XCD tile order on this handle for
- * `steps` steps each, keeps the fastest and appends it to the JSON-lines
- * cache at cache_path (NULL: no cache; a cache hit skips the timing).  All
- * shapes give bit-identical results.  The handle is re-initialised after
- * tuning.  chosen[3] (may be NULL) receives block, cells per thread, XCD
- * order.  Fused engine only. */
+ * workgroup size x cells per thread x XCD tile order x load variant on this
+ * handle for `steps` steps each, keeps the fastest and appends it to the
+ * JSON-lines cache at cache_path (NULL: no cache; a cache hit skips the
+ * timing).  All shapes give bit-identical results.  The handle is
+ * re-initialised after tuning.  chosen[4] (may be NULL) receives block, cells
+ * per thread, XCD order, non-temporal loads.  Fused engine only. */
 int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t *chosen);
+/* Set the launch shape by hand (block 64..512 step 64, cells per thread 1|2,
+ * XCD order 0|1, non-temporal loads 0|1); results do not depend on it. */
+int imexlb_set_launch(imexlb_handle *h, int block, int cells_per_thread, int xcd_order, int nt_loads);
 
 /* ---- batched ensemble (no counterpart in the reference) ----------------
  * Many independent domains -- TGV boxes and config-mode cavities of uneven
