@@ -95,6 +95,8 @@ SIGNATURES = [
     ("imexlb_copy_bandwidth", _i, [_i, ctypes.c_size_t, _i, _dp, _fp]),
     ("imexlb_system_read", _i, [ctypes.c_char_p, _i, _i, ctypes.POINTER(SystemC)]),
     ("imexlb_comm_layout", _i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(Comm)]),
+    ("imexlb_checkpoint_write", _i, [_vp, ctypes.c_char_p]),
+    ("imexlb_checkpoint_read", _i, [_vp, ctypes.c_char_p]),
     ("imexlb_tecplot_write", _i, [ctypes.c_char_p, ctypes.c_double, _i, _i, _i, _dp, _dp, _dp, _dp]),
     ("imexlb_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_set_launch", _i, [_vp, _i, _i, _i, _i]),

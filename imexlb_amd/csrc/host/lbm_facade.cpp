@@ -127,6 +127,8 @@ void LBM::sync() { check(imexlb_sync(h_)); }
 void LBM::moments(double *p, double *u, double *v, double *w) { check(imexlb_get_moments(h_, p, u, v, w)); }
 void LBM::diagnostics(double &ke, double &mass) { check(imexlb_diagnostics(h_, &ke, &mass)); }
 void LBM::autotune(const char *cache) { check(imexlb_autotune(h_, cache, 10, nullptr)); }
+void LBM::checkpoint(const std::string &path) { check(imexlb_checkpoint_write(h_, path.c_str())); }
+void LBM::restart(const std::string &path) { check(imexlb_checkpoint_read(h_, path.c_str())); }
 imexlb_info LBM::info()
 {
     imexlb_info i;

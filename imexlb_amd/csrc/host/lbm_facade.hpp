@@ -74,6 +74,8 @@ struct LBM {
     void moments(double *p, double *u, double *v, double *w);
     void diagnostics(double &ke, double &mass);
     void autotune(const char *cache);
+    void checkpoint(const std::string &path);
+    void restart(const std::string &path);
     imexlb_info info();
 
     CommHelper comm;

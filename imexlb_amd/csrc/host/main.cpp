@@ -11,6 +11,8 @@
 //          [--lid U] [--sphere cx,cy,cz,r] [--tune CACHE] [--device D]
 //          [--output]   (Tecplot output<n>.plt every inter steps, as the
 //                        cavity driver does: lid_driven_cavity/main.cpp:26,39-44)
+//          [--restart FILE] (resume from a restart file; no warm-up)
+//          [--checkpoint FILE] (write a restart file after the last step)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -41,7 +43,7 @@ int main(int argc, char **argv)
     std::string input = "input.in";
     int steps = -1;
     bool honour_input = false, fast = false, have_sphere = false, output = false;
-    std::string tune;
+    std::string tune, restart, checkpoint;
     for (int a = 1; a < argc; ++a) {
         std::string k = argv[a];
         auto val = [&]() -> std::string {
@@ -64,6 +66,8 @@ int main(int argc, char **argv)
         else if (k == "--lid") opt.lid_u = std::atof(val().c_str());
         else if (k == "--device") opt.device = std::atoi(val().c_str());
         else if (k == "--tune") tune = val();
+        else if (k == "--restart") restart = val();
+        else if (k == "--checkpoint") checkpoint = val();
         else if (k == "--sphere") {
             std::string s = val();
             if (std::sscanf(s.c_str(), "%lf,%lf,%lf,%lf", &opt.sphere[0], &opt.sphere[1], &opt.sphere[2],
@@ -100,7 +104,10 @@ int main(int argc, char **argv)
         l1.setup_subdomain();
         std::cout << "Currently using HIP device: " << opt.device << std::endl;
 
-        if (opt.case_id == IMEXLB_CASE_TGV) {
+        if (!restart.empty()) {
+            l1.restart(restart);
+            if (l1.comm.me == 0) std::cout << "Restarted from " << restart << std::endl;
+        } else if (opt.case_id == IMEXLB_CASE_TGV) {
             if (l1.comm.me == 0) std::cout << "Warm-Up" << std::endl;
             for (int it = 1; it <= 20; it++) {  // tgv/main.cpp:51-62
                 l1.Collision();
@@ -176,6 +183,10 @@ int main(int argc, char **argv)
             double ke = 0, mass = 0;
             l1.diagnostics(ke, mass);
             std::printf("Final KE = %.15e  mass = %.15e\n", ke, mass);
+        }
+        if (!checkpoint.empty()) {
+            l1.checkpoint(checkpoint);
+            std::printf("Restart file %s written\n", checkpoint.c_str());
         }
     } catch (const Error &e) {
         std::fprintf(stderr, "imexlb: %s (status %d)\n", e.what(), e.status);

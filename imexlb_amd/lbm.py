@@ -142,6 +142,12 @@ class LBM:
         self._chk(lib().imexlb_sphere_force(self.h, dptr(out)))
         return out
 
+    def checkpoint(self, path):
+        self._chk(lib().imexlb_checkpoint_write(self.h, str(path).encode()))
+
+    def restart(self, path):
+        self._chk(lib().imexlb_checkpoint_read(self.h, str(path).encode()))
+
     def MPIoutput(self, n, directory="."):
         """Write output<n>.plt (Tecplot, x y z u v w p) as the reference's
         LBM::MPIoutput does (tgv/src/lbm.cpp:335-669)."""

@@ -138,6 +138,14 @@ int imexlb_get_moments(imexlb_handle *h, double *p, double *u, double *v, double
 int imexlb_get_f(imexlb_handle *h, double *f);
 int imexlb_set_f(imexlb_handle *h, const double *f);
 
+/* Restart file (no counterpart in the reference, which writes only the
+ * macroscopic fields): "IMXLBCK1", the creating configuration, the step
+ * count, then the post-collision populations [i][z][y][x] as FP64.  Reading
+ * needs a handle of the same case, mode and extents (storage type and
+ * precision may differ) and resumes at the saved step.  Fused engine. */
+int imexlb_checkpoint_write(imexlb_handle *h, const char *path);
+int imexlb_checkpoint_read(imexlb_handle *h, const char *path);
+
 /* FP64 kinetic energy 0.5 sum |u|^2 and mass sum 3p over the fluid cells of
  * the last moments written. */
 int imexlb_diagnostics(imexlb_handle *h, double *ke, double *mass);
