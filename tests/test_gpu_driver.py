@@ -50,3 +50,17 @@ def test_driver_sphere_and_fast_path(tmp_path):
     ke1, _, _ = run(tmp_path, text, ["--case", "tgv", "--honour-input"])
     ke2, _, out = run(tmp_path, text, ["--case", "tgv", "--honour-input", "--fast"])
     assert ke1 == ke2
+
+
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_driver_restart_continues_the_run(tmp_path, storage):
+    """30 steps, a restart file, then 11 more steps from it, end where one run
+    of 41 steps ends (odd totals exercise the AA layout parity)."""
+    text = "1.0  0.1  100\n0.05  41  0\n32 16 24\n"
+    common = ["--case", "tgv", "--storage", storage, "--honour-input"]
+    ke_full, mass_full, _ = run(tmp_path, text, common)
+    ck = str(tmp_path / "run.ck")
+    run(tmp_path, text, common + ["--steps", "30", "--checkpoint", ck])
+    ke, mass, out = run(tmp_path, text, common + ["--steps", "11", "--restart", ck])
+    assert "Restarted from" in out and "Warm-Up" not in out
+    assert (ke, mass) == (ke_full, mass_full)
