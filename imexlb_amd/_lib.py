@@ -100,6 +100,7 @@ SIGNATURES = [
     ("imexlb_tecplot_write", _i, [ctypes.c_char_p, ctypes.c_double, _i, _i, _i, _dp, _dp, _dp, _dp]),
     ("imexlb_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_set_launch", _i, [_vp, _i, _i, _i, _i]),
+    ("imexlb_tune_placement", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_ensemble_generate", _i, [ctypes.c_uint64, _i, ctypes.POINTER(EnsembleDomain)]),
     ("imexlb_ensemble_create", _i, [ctypes.POINTER(EnsembleDomain), _i, _i, _i, _i, ctypes.POINTER(_vp)]),
     ("imexlb_ensemble_destroy", _i, [_vp]),

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -784,6 +785,63 @@ int imexlb_set_launch(imexlb_handle *h, int block, int cpt, int xcd, int ntl)
     h->xcd = xcd;
     h->ntl = ntl;
     return IMEXLB_OK;
+}
+
+int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per_step, int32_t *chosen)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (trials < 1 || trials > 16) return fail(h, IMEXLB_EINVAL, "placement trials %d must be in [1, 16]", trials);
+    if (h->ph) return fail(h, IMEXLB_EUNSUPPORTED, "the phased engine keeps its own layout");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (steps < 2) steps = 2;
+    steps += steps & 1;  // whole AA periods
+    const size_t lat_bytes = (size_t)Q * h->pstride * h->elem;
+    const int nlat = h->cfg.storage == IMEXLB_PULL ? 2 : 1;
+    std::vector<std::array<void *, 2>> cand{{h->lat[0], h->lat[1]}};
+    std::vector<float> ms(trials, 0.0f);
+    auto time_current = [&](float &out) -> int {
+        int r = imexlb_initialize(h);
+        if (!r) r = imexlb_step(h, 2, 0);
+        if (!r) r = imexlb_step_timed(h, steps, 0, &out);
+        if (!r) out /= (float)steps;
+        return r;
+    };
+    rc = time_current(ms[0]);
+    for (int t = 1; !rc && t < trials; ++t) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < nlat * lat_bytes + (2ull << 30)) break;
+        std::array<void *, 2> p{nullptr, nullptr};
+        bool ok = true;
+        for (int k = 0; k < nlat && ok; ++k) ok = hipMalloc(&p[k], lat_bytes) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            for (void *q : p)
+                if (q) hipFree(q);
+            break;
+        }
+        cand.push_back(p);
+        h->lat[0] = p[0];
+        h->lat[1] = p[1];
+        rc = time_current(ms[t]);
+    }
+    int best = 0;
+    if (!rc)
+        for (int t = 1; t < (int)cand.size(); ++t)
+            if (ms[t] < ms[best]) best = t;
+    (void)hipStreamSynchronize(h->stream);
+    for (int t = 0; t < (int)cand.size(); ++t)
+        if (t != best)
+            for (void *q : cand[t])
+                if (q) hipFree(q);
+    h->lat[0] = cand[best][0];
+    h->lat[1] = cand[best][1];
+    const int rc2 = imexlb_initialize(h);
+    if (rc) return rc;
+    if (ms_per_step)
+        for (int t = 0; t < trials; ++t) ms_per_step[t] = ms[t];
+    if (chosen) *chosen = best;
+    return rc2;
 }
 
 int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch)

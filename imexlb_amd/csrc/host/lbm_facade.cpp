@@ -127,6 +127,12 @@ void LBM::sync() { check(imexlb_sync(h_)); }
 void LBM::moments(double *p, double *u, double *v, double *w) { check(imexlb_get_moments(h_, p, u, v, w)); }
 void LBM::diagnostics(double &ke, double &mass) { check(imexlb_diagnostics(h_, &ke, &mass)); }
 void LBM::autotune(const char *cache) { check(imexlb_autotune(h_, cache, 10, nullptr)); }
+int LBM::tune_placement(int trials)
+{
+    int32_t kept = 0;
+    check(imexlb_tune_placement(h_, trials, 10, nullptr, &kept));
+    return kept;
+}
 void LBM::checkpoint(const std::string &path) { check(imexlb_checkpoint_write(h_, path.c_str())); }
 void LBM::restart(const std::string &path) { check(imexlb_checkpoint_read(h_, path.c_str())); }
 imexlb_info LBM::info()

@@ -74,6 +74,7 @@ struct LBM {
     void moments(double *p, double *u, double *v, double *w);
     void diagnostics(double &ke, double &mass);
     void autotune(const char *cache);
+    int tune_placement(int trials);  // index of the lattice kept
     void checkpoint(const std::string &path);
     void restart(const std::string &path);
     imexlb_info info();

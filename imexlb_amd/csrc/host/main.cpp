@@ -13,6 +13,7 @@
 //                        cavity driver does: lid_driven_cavity/main.cpp:26,39-44)
 //          [--restart FILE] (resume from a restart file; no warm-up)
 //          [--checkpoint FILE] (write a restart file after the last step)
+//          [--place N] (time N lattice allocations, keep the fastest)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -44,6 +45,7 @@ int main(int argc, char **argv)
     int steps = -1;
     bool honour_input = false, fast = false, have_sphere = false, output = false;
     std::string tune, restart, checkpoint;
+    int place = 1;
     for (int a = 1; a < argc; ++a) {
         std::string k = argv[a];
         auto val = [&]() -> std::string {
@@ -68,6 +70,7 @@ int main(int argc, char **argv)
         else if (k == "--tune") tune = val();
         else if (k == "--restart") restart = val();
         else if (k == "--checkpoint") checkpoint = val();
+        else if (k == "--place") place = std::atoi(val().c_str());
         else if (k == "--sphere") {
             std::string s = val();
             if (std::sscanf(s.c_str(), "%lf,%lf,%lf,%lf", &opt.sphere[0], &opt.sphere[1], &opt.sphere[2],
@@ -100,6 +103,7 @@ int main(int argc, char **argv)
         CommHelper comm(1, 0, opt.case_id == IMEXLB_CASE_TGV, s1.sx, s1.sy, s1.sz);
         LBM l1(comm, s1.sx, s1.sy, s1.sz, s1.tau, s1.rho0, s1.u0, opt);
         if (!tune.empty()) l1.autotune(tune.c_str());
+        if (place > 1) std::cout << "Lattice placement kept: " << l1.tune_placement(place) << " of " << place << std::endl;
         l1.Initialize();
         l1.setup_subdomain();
         std::cout << "Currently using HIP device: " << opt.device << std::endl;

@@ -167,6 +167,15 @@ class LBM:
     def set_launch(self, block, cells_per_thread=1, xcd_order=1, nt_loads=0):
         self._chk(lib().imexlb_set_launch(self.h, int(block), int(cells_per_thread), int(xcd_order), int(nt_loads)))
 
+    def tune_placement(self, trials=6, steps=10):
+        """Time the step on up to `trials` lattice allocations (each with its
+        own HBM pages) and keep the fastest; the state is re-initialised.
+        Returns (index kept, [ms per step of each candidate, 0 if none])."""
+        ms = (ctypes.c_float * int(trials))()
+        ch = ctypes.c_int32()
+        self._chk(lib().imexlb_tune_placement(self.h, int(trials), int(steps), ms, ctypes.byref(ch)))
+        return ch.value, list(ms)
+
     def info(self):
         i = L.Info()
         self._chk(lib().imexlb_get_info(self.h, ctypes.byref(i)))

@@ -173,6 +173,18 @@ int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t
 /* Set the launch shape by hand (block 64..512 step 64, cells per thread 1|2,
  * XCD order 0|1, non-temporal loads 0|1); results do not depend on it. */
 int imexlb_set_launch(imexlb_handle *h, int block, int cells_per_thread, int xcd_order, int nt_loads);
+/* Lattice placement search (no counterpart in the reference).  The step rate
+ * of a lattice depends by several percent on where its pages land in HBM,
+ * and stays fixed for the life of the allocation.  This allocates up to
+ * trials - 1 further lattices while device memory allows (2 GiB stay free),
+ * keeping each candidate until the end so that every one has its own pages,
+ * times `steps` steps on each with the current launch shape, keeps the
+ * fastest and frees the others.  ms_per_step (may be NULL; `trials`
+ * entries) receives each candidate's time per step, 0 where none was
+ * allocated; *chosen (may be NULL) the index kept, 0 being the original.
+ * The state is re-initialised.  Results do not depend on the placement.
+ * Fused engine only; trials in [1, 16]. */
+int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per_step, int32_t *chosen);
 
 /* ---- batched ensemble (no counterpart in the reference) ----------------
  * Many independent domains -- TGV boxes and config-mode cavities of uneven

@@ -107,3 +107,31 @@ def test_live_handle_rejects_bad_calls(tmp_path):
     o.Initialize()
     o.run(3)
     assert_bitexact(o, g)
+
+
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_placement_search_keeps_results(storage):
+    """The lattice placement search swaps in the fastest of several
+    allocations and re-initialises: the run that follows is the oracle's, bit
+    for bit, and bad arguments come back as errors."""
+    from imexlb_amd import LBM, ImexlbError
+    n = (64, 24, 20)
+    tau = tgv_tau(n[0])
+    g = LBM(*n, tau, u0=0.05, storage=storage)
+    g.Initialize()
+    g.step(3)
+    kept, ms = g.tune_placement(4, steps=4)
+    assert 0 <= kept < 4 and len(ms) == 4
+    assert all(m > 0 for m in ms) and ms[kept] == min(ms)
+    assert g.info()["steps"] == 0
+    g.step(9)
+    o = Oracle(TGV, *n, tau0=tau, u0=0.05)
+    o.Initialize()
+    o.run(9)
+    assert_bitexact(o, g)
+    for bad in (0, 17):
+        with pytest.raises(ImexlbError):
+            g.tune_placement(bad)
+    ph = LBM(16, 8, 8, tau, u0=0.05, engine="phased")
+    with pytest.raises(ImexlbError):
+        ph.tune_placement(2)
