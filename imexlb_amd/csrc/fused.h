@@ -141,6 +141,9 @@ __device__ __forceinline__ void collide_f64(double (&f)[Q], double pm, double u,
 // sum is kept in FP64, the rest in FP32.
 __device__ __forceinline__ void moments_lp(const float (&g)[Q], double &pd, float &u, float &v, float &w)
 {
+    // no contraction here either: every kernel variant that inlines this
+    // rounds the same way, so the launch shape never changes a bit
+#pragma clang fp contract(off)
     pd = 0.0;
     u = 0.f;
     v = 0.f;
@@ -159,6 +162,7 @@ __device__ __forceinline__ void moments_lp(const float (&g)[Q], double &pd, floa
 
 __device__ __forceinline__ void collide_lp(float (&g)[Q], double pd, float u, float v, float w, float omega)
 {
+#pragma clang fp contract(off)
     const float p3 = (float)pd;  // 3 p' = sum g
     const float u15 = 1.5f * (u * u + v * v + w * w);
 #pragma unroll

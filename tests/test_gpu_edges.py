@@ -135,3 +135,25 @@ def test_placement_search_keeps_results(storage):
     ph = LBM(16, 8, 8, tau, u0=0.05, engine="phased")
     with pytest.raises(ImexlbError):
         ph.tune_placement(2)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_reduced_precision_bits_do_not_depend_on_launch_shape(dtype, storage):
+    """FP32/FP16 storage: every launch shape gives the same bits (the FP32
+    collision is written without contraction, so no kernel variant rounds
+    differently)."""
+    from imexlb_amd import LBM
+    n = (192, 5, 6)
+    tau = tgv_tau(n[0])
+    out = []
+    for shape in SHAPES:
+        g = LBM(*n, tau, u0=0.05, dtype=dtype, storage=storage)
+        g.set_launch(*shape)
+        g.Initialize()
+        g.step(13)
+        out.append((g.get_f(), g.moments()))
+    for f, m in out[1:]:
+        assert np.array_equal(out[0][0], f)
+        for a, b in zip(out[0][1], m):
+            assert np.array_equal(a, b)
