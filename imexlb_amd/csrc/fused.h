@@ -139,37 +139,69 @@ __device__ __forceinline__ void collide_f64(double (&f)[Q], double pm, double u,
 // Reduced-precision storage holds g = f - w S.  The incompressible BGK update
 // is linear in p, so g obeys the same update with p' = p - S/3; the density
 // sum is kept in FP64, the rest in FP32.
+//
+// Both functions work on the 13 bounce-back pairs (i, bb(i)) = (2k+1, 2k+2),
+// whose velocities are opposite and whose weights are equal: the moments take
+// the pair sums and differences, and the equilibria of a pair share
+// 3 p' - 1.5 u.u + 4.5 (e.u)^2 and differ only in the sign of 3 e.u.  That
+// saves about a quarter of the vector operations per cell, which matters
+// where 108 bytes per cell leave little time for arithmetic (FP16).
+// Contraction is off and the fused multiply-adds are explicit, so every
+// kernel variant that inlines these rounds the same way and the launch shape
+// never changes a bit.
+constexpr int NPAIR = 13;
+constexpr bool pairs_adjacent()
+{
+    for (int k = 0; k < NPAIR; ++k) {
+        const int i = 2 * k + 1;
+        if (BB[i] != i + 1 || EX[i + 1] != -EX[i] || EY[i + 1] != -EY[i] || EZ[i + 1] != -EZ[i] ||
+            WCLASS[i + 1] != WCLASS[i])
+            return false;
+    }
+    return true;
+}
+static_assert(pairs_adjacent(), "the pair form needs bb(2k+1) = 2k+2 with opposite velocity and equal weight");
+
 __device__ __forceinline__ void moments_lp(const float (&g)[Q], double &pd, float &u, float &v, float &w)
 {
-    // no contraction here either: every kernel variant that inlines this
-    // rounds the same way, so the launch shape never changes a bit
 #pragma clang fp contract(off)
-    pd = 0.0;
+    pd = (double)g[0];
     u = 0.f;
     v = 0.f;
     w = 0.f;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        pd += (double)g[i];
-        if (EX[i] > 0) u += g[i];
-        if (EX[i] < 0) u -= g[i];
-        if (EY[i] > 0) v += g[i];
-        if (EY[i] < 0) v -= g[i];
-        if (EZ[i] > 0) w += g[i];
-        if (EZ[i] < 0) w -= g[i];
+    for (int k = 0; k < NPAIR; ++k) {
+        const int i = 2 * k + 1;  // bb(i) = i + 1, e_bb(i) = -e_i
+        const float sum = g[i] + g[i + 1];
+        const float dif = g[i] - g[i + 1];
+        pd += (double)sum;
+        if (EX[i] > 0) u += dif;
+        if (EX[i] < 0) u -= dif;
+        if (EY[i] > 0) v += dif;
+        if (EY[i] < 0) v -= dif;
+        if (EZ[i] > 0) w += dif;
+        if (EZ[i] < 0) w -= dif;
     }
 }
 
+// g <- g + omega (feq - g), feq = w (3 p' + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u).
+// The non-equilibrium part feq - g is formed first, as in the reference's
+// update (tgv/src/lbm.cpp:228-234); (1 - omega) g + omega feq would round
+// at the size of g and lose most of it at omega near 2.
 __device__ __forceinline__ void collide_lp(float (&g)[Q], double pd, float u, float v, float w, float omega)
 {
 #pragma clang fp contract(off)
-    const float p3 = (float)pd;  // 3 p' = sum g
-    const float u15 = 1.5f * (u * u + v * v + w * w);
+    const float a = (float)pd - 1.5f * (u * u + v * v + w * w);  // 3 p' - 1.5 u.u
+    g[0] = __builtin_fmaf(omega, (float)WVAL[0] * a - g[0], g[0]);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
+    for (int k = 0; k < NPAIR; ++k) {
+        const int i = 2 * k + 1;
         const float edu = edu_of<float>(i, u, v, w);
-        const float feq = (float)WVAL[WCLASS[i]] * (p3 + 3.f * edu + 4.5f * edu * edu - u15);
-        g[i] = g[i] + omega * (feq - g[i]);
+        const float even = __builtin_fmaf(4.5f * edu, edu, a);  // a + 4.5 (e.u)^2
+        const float odd = 3.f * edu;
+        const float wi = (float)WVAL[WCLASS[i]];
+        g[i] = __builtin_fmaf(omega, wi * (even + odd) - g[i], g[i]);
+        g[i + 1] = __builtin_fmaf(omega, wi * (even - odd) - g[i + 1], g[i + 1]);
     }
 }
 
