@@ -25,17 +25,25 @@ namespace imx {
 template <typename S> struct RealOf { using type = float; };
 template <> struct RealOf<double> { using type = double; };
 
+// Load the stored value at element `at` of the lattice, as a population of
+// weight class `cls` (FP16 storage is scaled per class).
 template <typename S, bool NT = false>
-__device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_t pstride, int i, uint32_t c)
+__device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t at, int cls)
 {
-    const S *p = lat + (uint64_t)i * pstride + c;
+    const S *p = lat + at;
     S x;
     if constexpr (NT) x = __builtin_nontemporal_load(p);
     else x = *p;
     if constexpr (std::is_same<S, _Float16>::value)
-        return (float)x * H_INV_SCALE[WCLASS[i]];
+        return (float)x * H_INV_SCALE[cls];
     else
         return x;
+}
+
+template <typename S, bool NT = false>
+__device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_t pstride, int i, uint32_t c)
+{
+    return ld_at<S, NT>(lat, (uint64_t)i * pstride + c, WCLASS[i]);
 }
 
 // NTL (non-temporal loads) variant of the gather: lines that no other
@@ -296,10 +304,13 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
             f[i] = ld_pop<S, NTL>(src, ps, i, c);
             if constexpr (BC == BC_CAVITY) {
                 // the odd step stored the bounced population raw; the lid
-                // term is added on arrival, as in the pull form
-                if (EZ[i] < 0 && k.z1) {
-                    if constexpr (F64) f[i] = f[i] + P.corr[i];
-                    else f[i] = f[i] + (float)P.corr[i];
+                // term is added on arrival, as in the pull form (a select:
+                // see the pull gather below)
+                if (EZ[i] < 0) {
+                    R lid;
+                    if constexpr (F64) lid = f[i] + P.corr[i];
+                    else lid = f[i] + (float)P.corr[i];
+                    f[i] = k.z1 ? lid : f[i];
                 }
             }
         }
@@ -310,29 +321,39 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
                 if (NTL && EX[i] == 0) f[i] = ld_pop<S, true>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
                 else f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
             } else if constexpr (BC == BC_CAVITY) {
-                if (!src_out(k, i)) {
-                    if (NTL && EX[i] == 0) f[i] = ld_pop<S, true>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
-                    else f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
-                } else {
-                    // halfway bounce-back: f_i(x) = f*_bb(i)(x) (+ lid term)
-                    R v = ld_pop<S>(src, ps, PAT == PAT_PULL ? BB[i] : i, c);
-                    if (EZ[i] < 0 && k.z1) {
-                        if constexpr (F64) v = v + P.corr[i];
-                        else v = v + (float)P.corr[i];
-                    }
-                    f[i] = v;
+                // Boundary choices are selects, not branches: a value loaded
+                // inside a divergent branch, or used by one, is waited for
+                // there, which would serialise the 27 loads of the wavefront.
+                // Halfway bounce-back where x - e_i is outside: f_i(x) = f*_bb(i)(x),
+                // plus the lid term on the moving wall.
+                const bool out = src_out(k, i);
+                const int s_in = PAT == PAT_PULL ? i : BB[i], s_out = PAT == PAT_PULL ? BB[i] : i;
+                const uint64_t at = out ? (uint64_t)s_out * ps + c : (uint64_t)s_in * ps + src_off(k, i);
+                R v;
+                if (NTL && EX[i] == 0) v = ld_at<S, true>(src, at, WCLASS[i]);
+                else v = ld_at<S>(src, at, WCLASS[i]);
+                if (EZ[i] < 0) {  // x - e_i outside whenever z is the top layer
+                    R lid;
+                    if constexpr (F64) lid = v + P.corr[i];
+                    else lid = v + (float)P.corr[i];
+                    v = k.z1 ? lid : v;
                 }
+                f[i] = v;
             } else {  // channel, pull only
-                if (EX[i] > 0 && k.x0) {
-                    if constexpr (F64) f[i] = P.feq_in[i];
-                    else f[i] = (float)P.feq_in[i];
-                } else {
-                    const uint32_t so = src_off(k, i);
-                    bool solid = false;
-                    if (k.m == 2 && i != 0) solid = P.mask[so] == 1;
-                    if (!solid) f[i] = ld_pop<S>(src, ps, i, so);
-                    else f[i] = ld_pop<S>(src, ps, BB[i], c);
+                // selects as above; only the mask byte of a neighbour of a
+                // solid cell is loaded under a branch (its value picks the address)
+                const uint32_t so = src_off(k, i);
+                bool solid = false;
+                if (i != 0 && k.m == 2) solid = P.mask[so] == 1;
+                const uint64_t at = solid ? (uint64_t)BB[i] * ps + c : (uint64_t)i * ps + so;
+                R v = ld_at<S>(src, at, WCLASS[i]);
+                if (EX[i] > 0) {
+                    R in;
+                    if constexpr (F64) in = P.feq_in[i];
+                    else in = (float)P.feq_in[i];
+                    v = k.x0 ? in : v;
                 }
+                f[i] = v;
             }
         }
     }
