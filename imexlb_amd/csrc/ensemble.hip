@@ -53,14 +53,29 @@ __device__ __forceinline__ void tile_run(const FusedParams &P, uint32_t first, b
     }
 }
 
+// The tile and domain tables are not written while the kernel runs: read
+// them through the constant address space, so that they come in by scalar
+// loads, once per workgroup and tile, instead of vector loads that every use
+// of a parameter would wait for.  (The host pass only checks the kernel's
+// source; the address space exists on the device side.)
+template <typename T>
+__device__ __forceinline__ T read_const(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T *)p;
+#else
+    return *p;
+#endif
+}
+
 // Persistent over the tile table: every workgroup exits after its last tile.
 template <typename S, int PAT, int CPT>
 __global__ void __launch_bounds__(512) k_ensemble(const EnsParams EP)
 {
     const bool wm = EP.write_moments != 0;
     for (uint32_t t = blockIdx.x; t < EP.ntiles; t += gridDim.x) {
-        const uint2 tl = EP.tiles[t];
-        const FusedParams &P = EP.doms[2 * tl.x + (uint32_t)EP.parity];
+        const uint2 tl = read_const(EP.tiles + t);
+        const FusedParams P = read_const(EP.doms + 2 * tl.x + (uint32_t)EP.parity);
         // non-temporal loads of unshared lines (measured faster on every
         // storage type of the single-domain kernel)
         if (P.bc == BC_PERIODIC) tile_run<S, BC_PERIODIC, PAT, CPT, true>(P, tl.y, wm);

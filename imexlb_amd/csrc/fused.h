@@ -22,6 +22,17 @@
 
 namespace imx {
 
+// Device memory through the global address space.  The single-domain
+// kernels get their pointers as kernel arguments, where the compiler infers
+// this by itself; the ensemble kernel reads them from its domain table, and
+// without the cast it would emit flat accesses, which also count against the
+// scalar-load counter and so are waited for at every parameter load.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *glb(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 template <typename S> struct RealOf { using type = float; };
 template <> struct RealOf<double> { using type = double; };
 
@@ -30,7 +41,7 @@ template <> struct RealOf<double> { using type = double; };
 template <typename S, bool NT = false>
 __device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t at, int cls)
 {
-    const S *p = lat + at;
+    const auto *p = glb(lat + at);
     S x;
     if constexpr (NT) x = __builtin_nontemporal_load(p);
     else x = *p;
@@ -61,7 +72,7 @@ __device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_
 template <typename S, bool NT = true>
 __device__ __forceinline__ void st_pop(S *lat, uint64_t pstride, int i, uint32_t c, typename RealOf<S>::type v)
 {
-    S *p = lat + (uint64_t)i * pstride + c;
+    auto *p = glb(lat + (uint64_t)i * pstride + c);
     S x;
     if constexpr (std::is_same<S, _Float16>::value) x = (_Float16)(v * H_SCALE[WCLASS[i]]);
     else x = v;
@@ -237,7 +248,7 @@ __device__ __forceinline__ void cell_setup(const FusedParams &P, uint32_t c, Cel
     k.m = 0;
     if constexpr (BC == BC_CHANNEL) {
         if (k.live) {
-            k.m = P.mask[c];
+            k.m = *glb(P.mask + c);
             if (k.m == 1) k.live = false;  // solid: nothing streams from or to it
         }
     }
@@ -344,7 +355,7 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
                 // solid cell is loaded under a branch (its value picks the address)
                 const uint32_t so = src_off(k, i);
                 bool solid = false;
-                if (i != 0 && k.m == 2) solid = P.mask[so] == 1;
+                if (i != 0 && k.m == 2) solid = *glb(P.mask + so) == 1;
                 const uint64_t at = solid ? (uint64_t)BB[i] * ps + c : (uint64_t)i * ps + so;
                 R v = ld_at<S>(src, at, WCLASS[i]);
                 if (EX[i] > 0) {
@@ -369,10 +380,10 @@ __device__ __forceinline__ void cell_collide(const FusedParams &P, const CellCtx
         double pm, um, vm, wm;
         moments_f64(f, pm, um, vm, wm);
         if (write_moments) {
-            P.mom_p[c] = pm;
-            P.mom_u[c] = um;
-            P.mom_v[c] = vm;
-            P.mom_w[c] = wm;
+            *glb(P.mom_p + c) = pm;
+            *glb(P.mom_u + c) = um;
+            *glb(P.mom_v + c) = vm;
+            *glb(P.mom_w + c) = wm;
         }
         collide_f64(f, pm, um, vm, wm, P.tau0);
     } else {
@@ -380,10 +391,10 @@ __device__ __forceinline__ void cell_collide(const FusedParams &P, const CellCtx
         float u, v, w;
         moments_lp(f, pd, u, v, w);
         if (write_moments) {
-            P.mom_p[c] = (pd + P.shift) / 3.0;
-            P.mom_u[c] = u;
-            P.mom_v[c] = v;
-            P.mom_w[c] = w;
+            *glb(P.mom_p + c) = (pd + P.shift) / 3.0;
+            *glb(P.mom_u + c) = u;
+            *glb(P.mom_v + c) = v;
+            *glb(P.mom_w + c) = w;
         }
         collide_lp(f, pd, u, v, w, P.omega_f);
     }
