@@ -65,7 +65,8 @@ __device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_
 // Store policy: 0 non-temporal everywhere, 1 plain everywhere, 2 (default)
 // non-temporal for aligned writes and plain for the AA odd step's x-shifted
 // writes, whose partial lines at segment ends then merge in L2 before they
-// reach HBM (measured: FP64 AA 512^3 from 11.9 to 14.2 GLUPS).
+// reach HBM (policy 0 measured slower on the FP64 AA 512^3 step; the odd
+// step's write traffic is 1.043 x its algorithmic bytes with policy 2).
 #ifndef IMX_STORE_POLICY
 #define IMX_STORE_POLICY 2
 #endif
