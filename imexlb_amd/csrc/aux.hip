@@ -79,6 +79,16 @@ __global__ void k_init(const InitParams P)
         const double f = feq_f64(i, p, u, v, w);
         put<S>(lat, P.pstride, P.layout == 1 ? BB[i] : i, c, f, P.shift);
     }
+    // the fields themselves are the moments of step 0, as the reference's
+    // Initialize leaves them in p, ua, va, wa; a solid cell reports a fluid at
+    // rest with p = 0, as every later step leaves it
+    if (P.mom[0]) {
+        const bool solid = P.mask && P.mask[c] == 1;
+        P.mom[0][c] = solid ? 0.0 : p;
+        P.mom[1][c] = u;
+        P.mom[2][c] = v;
+        P.mom[3][c] = w;
+    }
 }
 
 hipError_t launch_init(int dtype, const InitParams &P, hipStream_t st)

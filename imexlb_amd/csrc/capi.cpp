@@ -428,6 +428,7 @@ int imexlb_initialize(imexlb_handle *h)
     P.p0 = c.rho0 * P.cs2;
     P.shift = h->shift;
     P.mask = h->mask;
+    for (int k = 0; k < 4; ++k) P.mom[k] = h->mom[k];
     P.divx.init(c.nx);
     P.divy.init(c.ny);
     HIPCHK(h, launch_init(c.dtype, P, h->stream));
@@ -439,7 +440,7 @@ int imexlb_initialize(imexlb_handle *h)
                                  h->stream));
     }
     h->steps = 0;
-    h->mom_step = -1;
+    h->mom_step = 0;  // k_init wrote the initial fields
     h->initialized = true;
     h->phase_collided = false;
     return IMEXLB_OK;

@@ -421,13 +421,14 @@ int imexlb_ensemble_initialize(imexlb_ensemble *e)
         P.cs2 = 1.0 / 3.0;
         P.p0 = D.rho0 * P.cs2;
         P.shift = F.shift;
+        for (int k = 0; k < 4; ++k) P.mom[k] = e->mom[4 * d + k];
         P.divx.init(D.nx);
         P.divy.init(D.ny);
         ECHK(e, launch_init(e->dtype, P, e->stream));
     }
     ECHK(e, hipStreamSynchronize(e->stream));
     e->steps = 0;
-    e->mom_step = -1;
+    e->mom_step = 0;  // k_init wrote the initial fields
     return IMEXLB_OK;
 }
 

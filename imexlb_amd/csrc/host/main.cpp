@@ -126,6 +126,9 @@ int main(int argc, char **argv)
             l1.Initialize();  // tgv/main.cpp:68
         }
 
+        // the initial fields, as the cavity driver writes them (lid_driven_cavity/main.cpp:26)
+        if (output) l1.MPIoutput(0);
+
         double t_col = 0, t_pack = 0, t_exch = 0, t_unpack = 0, t_stream = 0, t_update = 0;
         double start = 0, end = 0;
         if (fast) {

@@ -72,6 +72,8 @@ struct InitParams {
     double u0, rho0, cs2, p0;
     double shift;
     const uint8_t *mask;
+    double *mom[4];          // p, u, v, w of the initial fields (solid cells: 0);
+                             // null: not written
     FastDiv divx, divy;
 };
 hipError_t launch_init(int dtype, const InitParams &P, hipStream_t st);

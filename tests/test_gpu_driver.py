@@ -64,3 +64,27 @@ def test_driver_restart_continues_the_run(tmp_path, storage):
     ke, mass, out = run(tmp_path, text, common + ["--steps", "11", "--restart", ck])
     assert "Restarted from" in out and "Warm-Up" not in out
     assert (ke, mass) == (ke_full, mass_full)
+
+
+def test_driver_output_files(tmp_path):
+    """--output writes output0.plt (the initial fields) and one file every
+    `inter` steps, as the cavity driver does (lid_driven_cavity/main.cpp:26,
+    39-44); the last holds the moments of the last step."""
+    import numpy as np
+    from imexlb_amd import LBM, System
+    text = "1.0  0.1  10\n0.1  20  10\n24 20 16\n"
+    run(tmp_path, text, ["--case", "cavity", "--mode", "config", "--output"])
+    n = 24 * 20 * 16
+    for k in range(3):
+        raw = (tmp_path / f"output{k}.plt").read_bytes()
+        assert len(raw) == 332 + 7 * n * 8 and raw[:8] == b"#!TDV112"
+    data0 = np.frombuffer((tmp_path / "output0.plt").read_bytes()[332:], "<f8").reshape(7, 16, 20, 24)
+    assert np.all(data0[3:6] == 0.0) and np.all(data0[6] == 1.0 / 3.0)  # rest at p = rho0 / 3
+    s = System(str(tmp_path / "input.in"), "cavity")
+    g = LBM(s.sx, s.sy, s.sz, s.tau, s.rho0, s.u0, case="cavity", mode="config")
+    g.Initialize()
+    g.step(20)
+    p, u, v, w = g.moments()
+    data = np.frombuffer((tmp_path / "output2.plt").read_bytes()[332:], "<f8").reshape(7, 16, 20, 24)
+    for k, a in enumerate((u, v, w, p)):
+        assert np.array_equal(data[3 + k], a)

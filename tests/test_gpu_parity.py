@@ -190,3 +190,39 @@ def test_checkpoint_restart_continues_bit_for_bit(tmp_path, storage, dtype):
     c = LBM(n[0] + 2, n[1], n[2], tau)
     with pytest.raises(Exception):
         c.restart(path)
+
+
+INIT_CASES = [(TGV, FAITHFUL, "fused", "pull", "f64"), (TGV, FAITHFUL, "fused", "aa", "f64"),
+              (TGV, FAITHFUL, "fused", "aa", "f16"), (TGV, FAITHFUL, "fused", "pull", "f32"),
+              (LDC, CONFIG, "fused", "aa", "f64"), (SPHERE, CONFIG, "fused", "pull", "f64"),
+              (TGV, FAITHFUL, "phased", "pull", "f64"), (LDC, FAITHFUL, "phased", "pull", "f64"),
+              (LDC, CONFIG, "phased", "pull", "f64"), (SPHERE, FAITHFUL, "phased", "pull", "f64"),
+              (SPHERE, CONFIG, "phased", "pull", "f64")]
+
+
+@pytest.mark.parametrize("case,mode,engine,storage,dtype", INIT_CASES)
+def test_initial_moments_are_the_reference_fields(case, mode, engine, storage, dtype):
+    """Right after Initialize the moments are the initial fields the
+    reference's Initialize leaves in p, ua, va, wa (what its cavity driver
+    writes out as output0 before the first step), in every storage type.
+    A solid cell of the channel reports p = 0, as every later step leaves it."""
+    if case == SPHERE and mode == FAITHFUL:
+        n, tau, u0, sph = (40, 14, 12), 0.08, 0.1, (0.0, 0.0, 0.0, 0.0)
+    elif case == SPHERE:
+        n, tau, u0, sph = (30, 14, 16), 0.05, 0.05, (10.0, 7.0, 8.0, 3.5)
+    elif case == LDC:
+        n, tau, u0, sph = (12, 10, 14), 0.1, 0.0 if mode == CONFIG else 0.1, (0.0, 0.0, 0.0, 0.0)
+    else:
+        n, tau, u0, sph = (16, 12, 20), tgv_tau(16, 100), 0.05, (0.0, 0.0, 0.0, 0.0)
+    o, g = make_pair(case, mode, n, tau, u0, engine=engine, storage=storage, dtype=dtype, sphere=sph)
+    mask = None
+    if case == SPHERE and mode == CONFIG and engine == "fused":
+        _, _, _, _, usr = o.fields_full()
+        mask = usr[3:-3, 3:-3, 3:-3].transpose(2, 1, 0) == 1
+        p = g.moments()[0]
+        assert np.all(p[~mask] == 0.0)
+    moments_equal(o, g, mask)
+    ke_o, m_o = o.diagnostics()
+    ke_g, m_g = g.diagnostics()
+    if mask is None:
+        assert abs(ke_g - ke_o) <= 1e-13 * max(ke_o, 1e-300) and abs(m_g - m_o) <= 1e-13 * max(abs(m_o), 1.0)
