@@ -58,3 +58,28 @@ def test_sphere_channel_fullsize_runs():
     f = g.sphere_force()
     assert f[0] > 0  # drag points downstream
     assert abs(f[1]) < 1e-6 * f[0] and abs(f[2]) < 1e-6 * f[0]
+
+
+def test_tgv512_reduced_precision_within_stated_error():
+    """TGV 512^3 (Re 1600, U0 0.05, AA): after 200 steps the velocity L2 of
+    FP16- and FP32-shifted storage against FP64 is within the stated bound
+    (FP16 <= 1e-3, FP32 <= 1e-5), and every run keeps its mass."""
+    from imexlb_amd import LBM
+    n = 512
+    tau = 0.05 * n / 1600.0 / (1 / 3) / 2 / PI_R
+    vel = {}
+    for dt in ("f64", "f32", "f16"):
+        g = LBM(n, n, n, tau, dtype=dt, storage="aa")
+        g.Initialize()
+        _, m0 = g.diagnostics()
+        g.step(200)
+        _, m1 = g.diagnostics()
+        assert abs(m1 - m0) <= (1e-12 if dt == "f64" else 1e-6) * m0
+        vel[dt] = g.moments()[1:]
+        g.close()
+
+    def l2(a, b):
+        return (sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b)) / sum(float(np.sum(y ** 2)) for y in b)) ** 0.5
+    e16, e32 = l2(vel["f16"], vel["f64"]), l2(vel["f32"], vel["f64"])
+    print(f"[tgv512] velocity L2 vs FP64 after 200 steps: FP32 {e32:.3e}, FP16 {e16:.3e}", flush=True)
+    assert e16 <= 1e-3 and e32 <= 1e-5

@@ -130,7 +130,8 @@ def test_sphere_config_fused_bitexact():
     assert np.abs(fg - fo).max() <= 1e-12 * np.abs(fo).max()
 
 
-@pytest.mark.parametrize("dtype,tol", [("f32", 2e-5), ("f16", 2e-2)])
+# measured: FP32 8.1e-7, FP16 3.6e-4; FP16 is held to the stated TGV bound 1e-3
+@pytest.mark.parametrize("dtype,tol", [("f32", 5e-6), ("f16", 1e-3)])
 @pytest.mark.parametrize("storage", ["pull", "aa"])
 def test_tgv_reduced_precision(dtype, tol, storage):
     n = (32, 32, 32)
@@ -142,8 +143,12 @@ def test_tgv_reduced_precision(dtype, tol, storage):
     vel_o = np.stack([uo, vo, wo])
     vel_g = np.stack([ug, vg, wg])
     l2 = np.linalg.norm(vel_g - vel_o) / np.linalg.norm(vel_o)
+    print(f"[lowp] tgv {storage} {dtype}: velocity L2 {l2:.3e}", flush=True)
     assert l2 < tol, l2
-    assert np.abs(pg - po).max() < 50 * tol * 0.05 ** 2 + 1e-6
+    dp = np.abs(pg - po).max()
+    print(f"[lowp] tgv {storage} {dtype}: max |dp| {dp:.3e}", flush=True)
+    # measured: FP32 1.1e-8, FP16 1.3e-5 (the step is deterministic)
+    assert dp < (5e-8 if dtype == "f32" else 5e-5)
 
 
 def test_diagnostics_match_oracle():
@@ -226,3 +231,27 @@ def test_initial_moments_are_the_reference_fields(case, mode, engine, storage, d
     ke_g, m_g = g.diagnostics()
     if mask is None:
         assert abs(ke_g - ke_o) <= 1e-13 * max(ke_o, 1e-300) and abs(m_g - m_o) <= 1e-13 * max(abs(m_o), 1.0)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+@pytest.mark.parametrize("case,storage", [(LDC, "pull"), (LDC, "aa"), (SPHERE, "pull")])
+def test_bounded_cases_reduced_precision(case, storage, dtype):
+    """Config cavity and sphere channel in FP32/FP16 storage against the FP64
+    oracle: the velocity field after 60 steps, relative L2 over fluid cells."""
+    if case == LDC:
+        n, tau, u0, sph = (20, 18, 16), 0.096, 0.0, (0.0, 0.0, 0.0, 0.0)
+    else:
+        n, tau, u0, sph = (40, 18, 20), 0.05, 0.05, (12.0, 9.0, 10.0, 4.0)
+    o, g = make_pair(case, CONFIG, n, tau, u0, storage=storage, dtype=dtype, sphere=sph)
+    o.run(60)
+    g.step(60)
+    vo = np.stack(o.moments()[1:])
+    vg = np.stack(g.moments()[1:])
+    if case == SPHERE:
+        _, _, _, _, usr = o.fields_full()
+        fluid = usr[3:-3, 3:-3, 3:-3].transpose(2, 1, 0) == 1
+        vo, vg = vo[:, fluid], vg[:, fluid]
+    l2 = np.linalg.norm(vg - vo) / np.linalg.norm(vo)
+    print(f"[lowp] case {case} {storage} {dtype}: velocity L2 {l2:.3e}", flush=True)
+    # measured: FP32 2.5e-7 (cavity) / 4.5e-7 (channel), FP16 1.7e-4 / 2.5e-4
+    assert l2 < (2e-6 if dtype == "f32" else 1e-3), l2
