@@ -38,7 +38,8 @@ constexpr int G = GHOST;
 struct Dims {
     int lx, ly, lz;
     int nx, ny, nz;
-    uint64_t ncell;  // lx * ly * lz
+    uint64_t ncell;  // lx * ly * lz, below 2^32 (checked at create)
+    FastDiv divx, divy;  // by lx and ly: cell index -> (i, j, k) without 64-bit division
     int case_id, mode;
     double tau0, rho0, u0, lid_u;
     double cx, cy, cz, r;
@@ -55,10 +56,12 @@ __host__ __device__ __forceinline__ uint64_t fidx(const Dims &d, int q, int i, i
 }
 __device__ __forceinline__ void decode(const Dims &d, uint64_t c, int &i, int &j, int &k)
 {
-    i = (int)(c % d.lx);
-    const uint64_t t = c / d.lx;
-    j = (int)(t % d.ly);
-    k = (int)(t / d.ly);
+    const uint32_t c32 = (uint32_t)c;
+    const uint32_t t = d.divx.div(c32);
+    i = (int)(c32 - t * (uint32_t)d.lx);
+    const uint32_t kk = d.divy.div(t);
+    j = (int)(t - kk * (uint32_t)d.ly);
+    k = (int)kk;
 }
 __device__ __forceinline__ bool interior(const Dims &d, int i, int j, int k)
 {
@@ -472,6 +475,8 @@ int phased_create(const imexlb_config &c, hipStream_t st, PhasedState **out, std
     d.ly = c.ny + 2 * G;
     d.lz = c.nz + 2 * G;
     d.ncell = (uint64_t)d.lx * d.ly * d.lz;
+    d.divx.init((uint32_t)d.lx);
+    d.divy.init((uint32_t)d.ly);
     d.case_id = c.case_id;
     d.mode = c.mode;
     d.tau0 = c.tau0;
@@ -484,6 +489,10 @@ int phased_create(const imexlb_config &c, hipStream_t st, PhasedState **out, std
     d.r = c.sphere_r;
     d.seed = c.seed;
     *out = s;
+    if (d.ncell >= (1ull << 32)) {
+        err = "the ghosted phased-engine lattice must hold fewer than 2^32 cells";
+        return IMEXLB_EUNSUPPORTED;
+    }
     if (c.case_id == IMEXLB_CASE_SPHERE && c.mode == IMEXLB_MODE_CONFIG) {
         const double r = c.sphere_r;
         if (!(r > 0) || c.sphere_cx - r < 2 || c.sphere_cx + r > c.nx - 3 || c.sphere_cy - r < 1 ||
