@@ -71,6 +71,29 @@ def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None, pl
     return g, ms, info
 
 
+def tgv_analytic_velocity(n, u0, re, steps):
+    """The Taylor-Green velocity of the initial field (tgv/src/lbm.cpp:197-200,
+    pi = 3.1415926) decayed viscously: |k|^2 = 3 (2 pi / n)^2, u(t) = u(0)
+    exp(-nu |k|^2 t), nu = tau0 / 3.  Valid while the nonlinear transfer is
+    small (t U0 k << 1)."""
+    nu = tgv_tau(n, re, u0) / 3.0
+    k = 2.0 * np.pi / n
+    decay = np.exp(-nu * 3.0 * k * k * steps)
+    a = np.arange(n) / n * 2.0 * PI_R
+    s, c = np.sin(a), np.cos(a)
+    # arrays indexed [z][y][x]
+    u = u0 * decay * s[None, None, :] * c[None, :, None] * c[:, None, None]
+    v = -u0 * decay * c[None, None, :] * s[None, :, None] * c[:, None, None]
+    return u, v
+
+
+def vel_l2_analytic(m, n, u0, re, steps):
+    u, v = tgv_analytic_velocity(n, u0, re, steps)
+    num = float(np.sum((m[1] - u) ** 2) + np.sum((m[2] - v) ** 2) + np.sum(m[3] ** 2))
+    den = float(np.sum(u ** 2) + np.sum(v ** 2))
+    return (num / den) ** 0.5
+
+
 def vel_l2(a, b):
     num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a[1:], b[1:]))
     den = sum(float(np.sum(y ** 2)) for y in b[1:])
@@ -149,6 +172,8 @@ def main():
     bytes_lup = info64["bytes_per_lup"]
     achieved = bytes_lup * cells / (per_launch_ms * 1e-3) / 1e9
     m64 = g64.moments()
+    an64 = vel_l2_analytic(m64, n, a.u0, a.re, W + K + 1)
+    log(f"[bench] FP64 velocity L2 vs the analytic decay after {W + K + 1} steps: {an64:.3e}")
     log(f"[bench] FP64 {glups64:.2f} GLUPS, {achieved:.0f} GB/s")
 
     extra = {}
@@ -159,10 +184,11 @@ def main():
             gl = cells * K / (ms * 1e-3) / 1e9
             m = g.moments()
             err = vel_l2(m, m64)
+            an = vel_l2_analytic(m, n, a.u0, a.re, W + K + 1)
             bw = info["bytes_per_lup"] * cells / (ms / K * 1e-3) / 1e9
             extra[f"{dt}_shifted"] = {"glups": gl, "ms_per_step": ms / K, "achieved_GBps": bw,
                                       "frac": bw / PEAK_GBS, "vs_f64": gl / glups64,
-                                      "vel_l2_vs_f64": err, "after_steps": W + K + 1,
+                                      "vel_l2_vs_f64": err, "vel_l2_vs_analytic": an, "after_steps": W + K + 1,
                                       "placement": info["placement"]}
             log(f"[bench] {dt} {gl:.2f} GLUPS ({gl / glups64:.2f}x FP64), L2 vs FP64 {err:.2e}")
             del g, m
@@ -203,6 +229,11 @@ def main():
                      + ("AA" if a.storage == "aa" else "pull") + ")",
                      "measured_copy_GBps": gbps_copy, "placement": info64["placement"]},
         "cpu_baseline": cpu,
+        "tgv_error": {"after_steps": W + K + 1, "f64_vel_l2_vs_analytic": an64,
+                      "note": "analytic = initial TGV field decayed by exp(-3 nu k^2 t), the linear solution; "
+                              "the departure grows as the nonlinear transfer, about 0.25 t U0 k (t U0 k = "
+                              f"{(W + K + 1) * a.u0 * 2 * np.pi / n:.3f} here). The FP32/FP16 lines carry their "
+                              "L2 against FP64 and against the same analytic field"},
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
