@@ -172,6 +172,7 @@ def main():
     bytes_lup = info64["bytes_per_lup"]
     achieved = bytes_lup * cells / (per_launch_ms * 1e-3) / 1e9
     m64 = g64.moments()
+    g64.close()  # its lattice is not needed again; frees room for the other storage types
     an64 = vel_l2_analytic(m64, n, a.u0, a.re, W + K + 1)
     log(f"[bench] FP64 velocity L2 vs the analytic decay after {W + K + 1} steps: {an64:.3e}")
     log(f"[bench] FP64 {glups64:.2f} GLUPS, {achieved:.0f} GB/s")
@@ -192,7 +193,7 @@ def main():
                                       "placement": info["placement"]}
             log(f"[bench] {dt} {gl:.2f} GLUPS ({gl / glups64:.2f}x FP64), L2 vs FP64 {err:.2e}")
             del g, m
-    del g64, m64
+    del m64
 
     gbps_copy, _ = copy_bandwidth(4 << 30, reps=10)
     log(f"[bench] device copy {gbps_copy:.0f} GB/s")
