@@ -508,10 +508,8 @@ int imexlb_step(imexlb_handle *h, int nsteps, int moments_every)
     int rc = set_device(h);
     if (rc) return rc;
     if (h->ph) {
-        for (int k = 0; k < nsteps; ++k) {
-            if ((rc = phased_full_step(h->ph, h->err))) return rc;
-            h->steps++;
-        }
+        if ((rc = phased_steps(h->ph, nsteps, h->err))) return rc;
+        h->steps += nsteps;
         return IMEXLB_OK;
     }
     for (int k = 0; k < nsteps; ++k) {

@@ -26,6 +26,8 @@ int phased_unpack(PhasedState *s, std::string &err);
 int phased_stream(PhasedState *s, std::string &err);
 int phased_update(PhasedState *s, std::string &err);
 int phased_full_step(PhasedState *s, std::string &err);
+// n full steps, fused where the phases allow; same results as n full steps
+int phased_steps(PhasedState *s, int n, std::string &err);
 int phased_get_moments(PhasedState *s, double *p, double *u, double *v, double *w, std::string &err);
 int phased_get_f(PhasedState *s, double *f, std::string &err);
 int phased_set_f(PhasedState *s, const double *f, std::string &err);

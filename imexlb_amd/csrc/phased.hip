@@ -370,6 +370,67 @@ __global__ void k_update(const Dims d, const Arrays a)
     a.p[c] = pm;
 }
 
+// Several steps at a time (imexlb_step): Streaming's pull, Update and the next
+// step's Collision in one pass per interior cell, from `src` (post-collision
+// f with its ghost shell already filled by the step's boundary kernels) into
+// `dst`.  The moments and their overrides are k_update's, the collision is
+// k_collide's, in the same operation order, so every value is the one the
+// phase-by-phase sequence produces; only the moments of ghost cells, which
+// nothing reads, are not written.  collide_next = 0 on the last step leaves
+// dst holding the streamed (pre-collision) populations, the state after
+// Update.  The channel's solid cells (config mode) keep their populations
+// and are left to k_stream2_keep.
+__global__ void k_stream_update_collide(const Dims d, const Arrays a, const double *__restrict__ src,
+                                        double *__restrict__ dst, int collide_next)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.ncell) return;
+    int i, j, k;
+    decode(d, c, i, j, k);
+    if (!interior(d, i, j, k)) return;
+    const bool solid_kept = d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG && a.usr[c] == 0;
+    double f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        f[q] = solid_kept ? src[(uint64_t)q * d.ncell + c] : src[fidx(d, q, i - EX[q], j - EY[q], k - EZ[q])];
+    double pm, um, vm, wm;
+    moments_f64(f, pm, um, vm, wm);
+    if (d.case_id == IMEXLB_CASE_SPHERE) {
+        const double m = (double)a.usr[c];
+        um = um * m;
+        vm = vm * m;
+        wm = wm * m;
+        pm = pm * m;
+    }
+    if (d.mode == IMEXLB_MODE_FAITHFUL) {
+        if (d.case_id == IMEXLB_CASE_CAVITY && k == G + d.nz - 1) um = 0.1;
+        if (d.case_id == IMEXLB_CASE_SPHERE) {
+            if (i == G) {
+                const int kz = k - G, glz = d.nz;
+                const double gm = (double)(glz - 1);
+                um = d.u0 * 4.0 * (double)kz * (double)(glz - 1 - kz) / (gm * gm);
+                vm = 0.0;
+                wm = 0.0;
+            }
+            if (i == d.lx - G - 1) pm = 0.0;
+        }
+    }
+    a.ua[c] = um;
+    a.va[c] = vm;
+    a.wa[c] = wm;
+    a.p[c] = pm;
+    if (solid_kept) return;
+    if (collide_next) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double feq = feq_ref(q, pm, um, vm, wm);
+            f[q] = f[q] - (f[q] - feq) / (d.tau0 + 0.5);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) dst[(uint64_t)q * d.ncell + c] = f[q];
+}
+
 __global__ void k_interior_io(const Dims d, double *f, double *buf, int nq, int to_buf)
 {
     const uint64_t n = (uint64_t)d.nx * d.ny * d.nz;
@@ -608,7 +669,8 @@ static void wall(PhasedState *s, int axis, int plane, int src_plane, int sign, i
     hipLaunchKernelGGL(k_wall, grid_for(n), dim3(256), 0, s->st, d, s->a, axis, plane, src_plane, sign, sa, sb);
 }
 
-int phased_stream(PhasedState *s, std::string &err)
+// Streaming's boundary kernels, in the reference's order, on s->a.f.
+static void boundaries(PhasedState *s)
 {
     const Dims &d = s->d;
     const int ls = G, le0 = G + d.nx, le1 = G + d.ny, le2 = G + d.nz;
@@ -641,6 +703,12 @@ int phased_stream(PhasedState *s, std::string &err)
             hipLaunchKernelGGL(k_channel_solid, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a);
         }
     }
+}
+
+int phased_stream(PhasedState *s, std::string &err)
+{
+    const Dims &d = s->d;
+    boundaries(s);
     hipLaunchKernelGGL(k_stream1, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a);
     hipLaunchKernelGGL(k_stream2_keep, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a.f, s->a.ft, s->a.usr);
     PH_CHK(hipGetLastError());
@@ -661,6 +729,29 @@ int phased_full_step(PhasedState *s, std::string &err)
     if ((rc = phased_collide(s, err)) || (rc = phased_pack(s, err)) || (rc = phased_exchange(s, err)) ||
         (rc = phased_unpack(s, err)) || (rc = phased_stream(s, err)) || (rc = phased_update(s, err)))
         return rc;
+    return IMEXLB_OK;
+}
+
+// n steps of Collision -> pack/exchange/unpack -> Streaming -> Update with
+// the same results as n calls of phased_full_step: the first Collision on its
+// own, then per step the halo and boundary kernels on the post-collision f,
+// one k_stream_update_collide pass (with the next step's Collision, except
+// after the last step) and the ghost-shell carry-over of k_stream2_keep.
+int phased_steps(PhasedState *s, int n, std::string &err)
+{
+    if (n <= 0) return IMEXLB_OK;
+    int rc = phased_collide(s, err);
+    if (rc) return rc;
+    const Dims &d = s->d;
+    for (int step = 1; step <= n; ++step) {
+        if ((rc = phased_unpack(s, err))) return rc;
+        boundaries(s);
+        hipLaunchKernelGGL(k_stream_update_collide, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a, s->a.f,
+                           s->a.ft, step < n ? 1 : 0);
+        hipLaunchKernelGGL(k_stream2_keep, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a.f, s->a.ft, s->a.usr);
+        PH_CHK(hipGetLastError());
+        std::swap(s->a.f, s->a.ft);
+    }
     return IMEXLB_OK;
 }
 
