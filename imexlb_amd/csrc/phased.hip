@@ -176,15 +176,32 @@ __global__ void k_periodic(const Dims d, const Arrays a, int px, int py, int pz)
     for (int q = 0; q < Q; ++q) a.f[(uint64_t)q * d.ncell + c] = a.f[(uint64_t)q * d.ncell + s];
 }
 
+// the planes y = le1, z = le2, x = ls0 - 1 and x = le0, one thread per cell
+// of each plane (cells on two planes are zeroed twice)
 __global__ void k_cavity_zero_planes(const Dims d, const Arrays a)
 {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= d.ncell) return;
+    const uint32_t nyz = (uint32_t)d.ly * d.lz, nxz = (uint32_t)d.lx * d.lz, nxy = (uint32_t)d.lx * d.ly;
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
-    decode(d, c, i, j, k);
-    if (j == G + d.ny || k == G + d.nz || i == G - 1 || i == G + d.nx)
+    if (t < 2u * nyz) {
+        const uint32_t p = t / nyz, r = t - p * nyz;
+        i = p == 0 ? G - 1 : G + d.nx;
+        k = (int)(r / d.ly);
+        j = (int)(r - (uint32_t)k * d.ly);
+    } else if ((t -= 2u * nyz) < nxz) {
+        j = G + d.ny;
+        k = (int)(t / d.lx);
+        i = (int)(t - (uint32_t)k * d.lx);
+    } else if ((t -= nxz) < nxy) {
+        k = G + d.nz;
+        j = (int)(t / d.lx);
+        i = (int)(t - (uint32_t)j * d.lx);
+    } else {
+        return;
+    }
+    const uint64_t c = cidx(d, i, j, k);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) a.f[(uint64_t)q * d.ncell + c] = 0.0;
+    for (int q = 0; q < Q; ++q) a.f[(uint64_t)q * d.ncell + c] = 0.0;
 }
 
 // Faithful wall bounce-back on one face: for the cells (a, b) of the plane in
@@ -321,6 +338,44 @@ __global__ void k_stream2_keep(const Dims d, const double *__restrict__ f_old, d
     decode(d, c, i, j, k);
     const bool solid_kept = d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG && usr[c] == 0;
     if (interior(d, i, j, k) && !solid_kept) return;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f_new[(uint64_t)q * d.ncell + c] = f_old[(uint64_t)q * d.ncell + c];
+}
+
+// The ghost shell only (every case but the config channel, which also keeps
+// its solid cells): thread t maps to the t-th non-interior cell, the two z
+// slabs first, then the two y slabs and the two x slabs between them.
+__host__ __device__ __forceinline__ uint64_t shell_cells(const Dims &d)
+{
+    return d.ncell - (uint64_t)d.nx * d.ny * d.nz;
+}
+
+__global__ void k_shell_keep(const Dims d, const double *__restrict__ f_old, double *__restrict__ f_new)
+{
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 >= shell_cells(d)) return;
+    uint32_t t = (uint32_t)t0;
+    const uint32_t lx = d.lx, ly = d.ly, nx = d.nx, ny = d.ny, nz = d.nz;
+    const uint32_t nzs = 2u * G * lx * ly, nys = 2u * G * lx * nz;
+    uint32_t i, j, k;
+    if (t < nzs) {
+        const uint32_t p = t / (lx * ly), r = t - p * (lx * ly);
+        k = p < (uint32_t)G ? p : nz + p;  // G + nz + (p - G)
+        j = r / lx;
+        i = r - j * lx;
+    } else if ((t -= nzs) < nys) {
+        const uint32_t p = t / (lx * nz), r = t - p * (lx * nz);
+        j = p < (uint32_t)G ? p : ny + p;
+        k = G + r / lx;
+        i = r - (k - G) * lx;
+    } else {
+        t -= nys;
+        const uint32_t p = t / (ny * nz), r = t - p * (ny * nz);
+        i = p < (uint32_t)G ? p : nx + p;
+        k = G + r / ny;
+        j = G + (r - (k - G) * ny);
+    }
+    const uint64_t c = cidx(d, (int)i, (int)j, (int)k);
 #pragma unroll
     for (int q = 0; q < Q; ++q) f_new[(uint64_t)q * d.ncell + c] = f_old[(uint64_t)q * d.ncell + c];
 }
@@ -654,7 +709,8 @@ int phased_unpack(PhasedState *s, std::string &err)
     if (d.case_id == IMEXLB_CASE_TGV)
         hipLaunchKernelGGL(k_periodic, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a, 1, 1, 1);
     else if (d.case_id == IMEXLB_CASE_CAVITY && d.mode == IMEXLB_MODE_FAITHFUL)
-        hipLaunchKernelGGL(k_cavity_zero_planes, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a);
+        hipLaunchKernelGGL(k_cavity_zero_planes, grid_for(2ull * d.ly * d.lz + (uint64_t)d.lx * d.lz + (uint64_t)d.lx * d.ly),
+                           dim3(256), 0, s->st, d, s->a);
     else if (d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG)
         hipLaunchKernelGGL(k_periodic, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a, 0, 1, 1);
     PH_CHK(hipGetLastError());
@@ -667,6 +723,16 @@ static void wall(PhasedState *s, int axis, int plane, int src_plane, int sign, i
     const int la = axis == 0 ? d.ly : d.lx, lb = axis == 2 ? d.ly : d.lz;
     const uint64_t n = (uint64_t)(la - 2 * (G - 1)) * (lb - 2 * (G - 1));
     hipLaunchKernelGGL(k_wall, grid_for(n), dim3(256), 0, s->st, d, s->a, axis, plane, src_plane, sign, sa, sb);
+}
+
+// Carry over what the copy-back leaves alone from s->a.f to s->a.ft.
+static void keep(PhasedState *s)
+{
+    const Dims &d = s->d;
+    if (d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG)
+        hipLaunchKernelGGL(k_stream2_keep, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a.f, s->a.ft, s->a.usr);
+    else
+        hipLaunchKernelGGL(k_shell_keep, grid_for(shell_cells(d)), dim3(256), 0, s->st, d, s->a.f, s->a.ft);
 }
 
 // Streaming's boundary kernels, in the reference's order, on s->a.f.
@@ -710,7 +776,7 @@ int phased_stream(PhasedState *s, std::string &err)
     const Dims &d = s->d;
     boundaries(s);
     hipLaunchKernelGGL(k_stream1, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a);
-    hipLaunchKernelGGL(k_stream2_keep, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a.f, s->a.ft, s->a.usr);
+    keep(s);
     PH_CHK(hipGetLastError());
     std::swap(s->a.f, s->a.ft);
     return IMEXLB_OK;
@@ -748,7 +814,7 @@ int phased_steps(PhasedState *s, int n, std::string &err)
         boundaries(s);
         hipLaunchKernelGGL(k_stream_update_collide, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a, s->a.f,
                            s->a.ft, step < n ? 1 : 0);
-        hipLaunchKernelGGL(k_stream2_keep, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a.f, s->a.ft, s->a.usr);
+        keep(s);
         PH_CHK(hipGetLastError());
         std::swap(s->a.f, s->a.ft);
     }
