@@ -150,11 +150,15 @@ __global__ void k_collide(const Dims d, const Arrays a)
     if (!interior(d, i, j, k)) return;
     if (d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG && a.usr[c] == 0) return;
     const double u = a.ua[c], v = a.va[c], w = a.wa[c], p = a.p[c];
+    // all 27 loads before any store: a store may alias a later load as far as
+    // the compiler knows, and would otherwise wait for each load in turn
+    double f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = a.f[(uint64_t)q * d.ncell + c];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const double feq = feq_ref(q, p, u, v, w);
-        double &fv = a.f[(uint64_t)q * d.ncell + c];
-        fv = fv - (fv - feq) / (d.tau0 + 0.5);
+        a.f[(uint64_t)q * d.ncell + c] = f[q] - (f[q] - feq) / (d.tau0 + 0.5);
     }
 }
 
@@ -171,9 +175,12 @@ __global__ void k_periodic(const Dims d, const Arrays a, int px, int py, int pz)
     if (py) jj = j < G ? j + d.ny : (j >= G + d.ny ? j - d.ny : j);
     if (pz) kk = k < G ? k + d.nz : (k >= G + d.nz ? k - d.nz : k);
     if (ii == i && jj == j && kk == k) return;
-    const uint64_t s = cidx(d, ii, jj, kk);
+    const uint64_t s = cidx(d, ii, jj, kk);  // an interior cell: never a target here
+    double v[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) a.f[(uint64_t)q * d.ncell + c] = a.f[(uint64_t)q * d.ncell + s];
+    for (int q = 0; q < Q; ++q) v[q] = a.f[(uint64_t)q * d.ncell + s];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.f[(uint64_t)q * d.ncell + c] = v[q];
 }
 
 // the planes y = le1, z = le2, x = ls0 - 1 and x = le0, one thread per cell
@@ -206,46 +213,71 @@ __global__ void k_cavity_zero_planes(const Dims d, const Arrays a)
 
 // Faithful wall bounce-back on one face: for the cells (a, b) of the plane in
 // [G-1, l-G+1)^2 and every direction pointing into the domain,
-// f_q(plane) = f_bb(q)(src_plane, a + sa e_a, b + sb e_b).
-__global__ void k_wall(const Dims d, const Arrays a, int axis, int plane, int src_plane, int sign, int sa, int sb)
+// f_q(plane) = f_bb(q)(src_plane, a + sa e_a, b + sb e_b).  Sources lie in
+// src_plane and targets in plane, so all loads are issued before the stores;
+// the axis and the inward sign are template arguments, which fixes the
+// directions taken at compile time (a runtime choice would put every load
+// under a branch that waits for it).
+template <int AXIS, int SIGN>
+__global__ void k_wall(const Dims d, const Arrays a, int plane, int src_plane, int sa, int sb)
 {
-    const int la = axis == 0 ? d.ly : d.lx;
-    const int lb = axis == 2 ? d.ly : d.lz;
+    const int la = AXIS == 0 ? d.ly : d.lx;
+    const int lb = AXIS == 2 ? d.ly : d.lz;
     const int na = la - 2 * (G - 1), nb = lb - 2 * (G - 1);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na * nb) return;
     const int A = G - 1 + t % na, B = G - 1 + t / na;
+    double v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int e[3] = {EX[q], EY[q], EZ[q]};
-        if (e[axis] * sign <= 0) continue;
-        int di, dj, dk, si, sj, sk;
-        if (axis == 0) {
-            di = plane; dj = A; dk = B;
+        if (e[AXIS] * SIGN <= 0) continue;
+        int si, sj, sk;
+        if (AXIS == 0) {
             si = src_plane; sj = A + sa * e[1]; sk = B + sb * e[2];
-        } else if (axis == 1) {
-            di = A; dj = plane; dk = B;
+        } else if (AXIS == 1) {
             si = A + sa * e[0]; sj = src_plane; sk = B + sb * e[2];
         } else {
-            di = A; dj = B; dk = plane;
             si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
         }
-        a.f[fidx(d, q, di, dj, dk)] = a.f[fidx(d, BB[q], si, sj, sk)];
+        v[q] = a.f[fidx(d, BB[q], si, sj, sk)];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e[3] = {EX[q], EY[q], EZ[q]};
+        if (e[AXIS] * SIGN <= 0) continue;
+        int di, dj, dk;
+        if (AXIS == 0) {
+            di = plane; dj = A; dk = B;
+        } else if (AXIS == 1) {
+            di = A; dj = plane; dk = B;
+        } else {
+            di = A; dj = B; dk = plane;
+        }
+        a.f[fidx(d, q, di, dj, dk)] = v[q];
     }
 }
 
-// sphere/src/lbm.cpp:326-347
-__global__ void k_sphere_inlet_outlet(const Dims d, const Arrays a, int outlet)
+// sphere/src/lbm.cpp:326-347.  Targets lie in the ghost plane, sources in
+// the first (inlet) or last (outlet) interior plane: loads first, then stores.
+template <int OUTLET>
+__global__ void k_sphere_inlet_outlet(const Dims d, const Arrays a)
 {
     const int na = d.ly - 2 * (G - 1), nb = d.lz - 2 * (G - 1);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na * nb) return;
     const int j = G - 1 + t % na, k = G - 1 + t / na;
     const int le0 = G + d.nx;
+    double v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        if (!outlet && EX[q] > 0) a.f[fidx(d, q, G - 1, j, k)] = a.f[fidx(d, q, G, j, k)];
-        if (outlet && EX[q] < 0) a.f[fidx(d, q, le0, j, k)] = a.f[fidx(d, q, le0 + EX[q], j + EY[q], k + EZ[q])];
+        if (!OUTLET && EX[q] > 0) v[q] = a.f[fidx(d, q, G, j, k)];
+        if (OUTLET && EX[q] < 0) v[q] = a.f[fidx(d, q, le0 + EX[q], j + EY[q], k + EZ[q])];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (!OUTLET && EX[q] > 0) a.f[fidx(d, q, G - 1, j, k)] = v[q];
+        if (OUTLET && EX[q] < 0) a.f[fidx(d, q, le0, j, k)] = v[q];
     }
 }
 
@@ -257,10 +289,20 @@ __global__ void k_sphere_usr(const Dims d, const Arrays a)
     int i, j, k;
     decode(d, c, i, j, k);
     if (!interior(d, i, j, k) || a.usr[c] != 0) return;
+    // every source (depth <= 2 from the interior) is loaded, the masks pick
+    // the stores: no load waits under a branch.  A source is never a target
+    // of this launch: between two solid voxels of a ball the midpoint voxel
+    // is solid too.
+    bool take[Q];
+    double v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        take[q] = a.usr[cidx(d, i + EX[q], j + EY[q], k + EZ[q])] == 1;
+        v[q] = a.f[fidx(d, BB[q], i + 2 * EX[q], j + 2 * EY[q], k + 2 * EZ[q])];
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q)
-        if (a.usr[cidx(d, i + EX[q], j + EY[q], k + EZ[q])] == 1)
-            a.f[fidx(d, q, i, j, k)] = a.f[fidx(d, BB[q], i + 2 * EX[q], j + 2 * EY[q], k + 2 * EZ[q])];
+        if (take[q]) a.f[fidx(d, q, i, j, k)] = v[q];
 }
 
 // config-mode cavity: halfway bounce-back as a fill of the depth-1 ghost shell
@@ -272,13 +314,17 @@ __global__ void k_cavity_halfway(const Dims d, const Arrays a, const double *lid
     decode(d, c, i, j, k);
     if (i < G - 1 || i > G + d.nx || j < G - 1 || j > G + d.ny || k < G - 1 || k > G + d.nz) return;
     if (interior(d, i, j, k)) return;
+    double v[Q], corr[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int xi = i + EX[q], xj = j + EY[q], xk = k + EZ[q];
-        if (!interior(d, xi, xj, xk)) continue;
-        double v = a.f[fidx(d, BB[q], xi, xj, xk)];
-        if (k == G + d.nz) v = v + lidcorr[q];
-        a.f[fidx(d, q, i, j, k)] = v;
+        v[q] = a.f[fidx(d, BB[q], i + EX[q], j + EY[q], k + EZ[q])];
+        corr[q] = lidcorr[q];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (!interior(d, i + EX[q], j + EY[q], k + EZ[q])) continue;
+        const double lid = v[q] + corr[q];
+        a.f[fidx(d, q, i, j, k)] = k == G + d.nz ? lid : v[q];
     }
 }
 
@@ -289,10 +335,16 @@ __global__ void k_channel_planes(const Dims d, const Arrays a, const double *feq
     if (t >= d.ly * d.lz) return;
     const int j = t % d.ly, k = t / d.ly;
     const int ls0 = G, le0 = G + d.nx;
+    double v[Q], in[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        a.f[fidx(d, q, ls0 - 1, j, k)] = feq_in[q];
-        a.f[fidx(d, q, le0, j, k)] = a.f[fidx(d, q, le0 - 1, j, k)];
+        v[q] = a.f[fidx(d, q, le0 - 1, j, k)];
+        in[q] = feq_in[q];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        a.f[fidx(d, q, ls0 - 1, j, k)] = in[q];
+        a.f[fidx(d, q, le0, j, k)] = v[q];
     }
 }
 
@@ -304,11 +356,17 @@ __global__ void k_channel_solid(const Dims d, const Arrays a)
     int i, j, k;
     decode(d, c, i, j, k);
     if (!interior(d, i, j, k) || a.usr[c] != 0) return;
+    bool take[Q];
+    double v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int xi = i + EX[q], xj = j + EY[q], xk = k + EZ[q];
-        if (a.usr[cidx(d, xi, xj, xk)] == 1) a.f[fidx(d, q, i, j, k)] = a.f[fidx(d, BB[q], xi, xj, xk)];
+        take[q] = a.usr[cidx(d, xi, xj, xk)] == 1;
+        v[q] = a.f[fidx(d, BB[q], xi, xj, xk)];
     }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (take[q]) a.f[fidx(d, q, i, j, k)] = v[q];
 }
 
 __global__ void k_stream1(const Dims d, const Arrays a)
@@ -318,8 +376,11 @@ __global__ void k_stream1(const Dims d, const Arrays a)
     int i, j, k;
     decode(d, c, i, j, k);
     if (!interior(d, i, j, k)) return;
+    double v[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) a.ft[fidx(d, q, i, j, k)] = a.f[fidx(d, q, i - EX[q], j - EY[q], k - EZ[q])];
+    for (int q = 0; q < Q; ++q) v[q] = a.f[fidx(d, q, i - EX[q], j - EY[q], k - EZ[q])];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.ft[fidx(d, q, i, j, k)] = v[q];
 }
 
 // The copy-back of Streaming (tgv/src/lbm.cpp:250-253: f = ft over the
@@ -722,7 +783,19 @@ static void wall(PhasedState *s, int axis, int plane, int src_plane, int sign, i
     const Dims &d = s->d;
     const int la = axis == 0 ? d.ly : d.lx, lb = axis == 2 ? d.ly : d.lz;
     const uint64_t n = (uint64_t)(la - 2 * (G - 1)) * (lb - 2 * (G - 1));
-    hipLaunchKernelGGL(k_wall, grid_for(n), dim3(256), 0, s->st, d, s->a, axis, plane, src_plane, sign, sa, sb);
+    const dim3 g = grid_for(n), b(256);
+#define IMX_WALL(AX, SG) hipLaunchKernelGGL((k_wall<AX, SG>), g, b, 0, s->st, d, s->a, plane, src_plane, sa, sb)
+    if (axis == 0) {
+        if (sign > 0) IMX_WALL(0, 1);
+        else IMX_WALL(0, -1);
+    } else if (axis == 1) {
+        if (sign > 0) IMX_WALL(1, 1);
+        else IMX_WALL(1, -1);
+    } else {
+        if (sign > 0) IMX_WALL(2, 1);
+        else IMX_WALL(2, -1);
+    }
+#undef IMX_WALL
 }
 
 // Carry over what the copy-back leaves alone from s->a.f to s->a.ft.
@@ -760,8 +833,8 @@ static void boundaries(PhasedState *s)
             wall(s, 2, ls - 1, ls + 1, +1, 2, 2);
             wall(s, 2, le2, le2 - 2, -1, 2, 2);
             const uint64_t n = (uint64_t)(d.ly - 2 * (G - 1)) * (d.lz - 2 * (G - 1));
-            hipLaunchKernelGGL(k_sphere_inlet_outlet, grid_for(n), dim3(256), 0, s->st, d, s->a, 0);
-            hipLaunchKernelGGL(k_sphere_inlet_outlet, grid_for(n), dim3(256), 0, s->st, d, s->a, 1);
+            hipLaunchKernelGGL(k_sphere_inlet_outlet<0>, grid_for(n), dim3(256), 0, s->st, d, s->a);
+            hipLaunchKernelGGL(k_sphere_inlet_outlet<1>, grid_for(n), dim3(256), 0, s->st, d, s->a);
             hipLaunchKernelGGL(k_sphere_usr, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a);
         } else {
             hipLaunchKernelGGL(k_channel_planes, grid_for((uint64_t)d.ly * d.lz), dim3(256), 0, s->st, d, s->a,
