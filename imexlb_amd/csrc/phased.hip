@@ -495,7 +495,7 @@ __global__ void k_update(const Dims d, const Arrays a)
 // nothing reads, are not written.  collide_next = 0 on the last step leaves
 // dst holding the streamed (pre-collision) populations, the state after
 // Update.  The channel's solid cells (config mode) keep their populations
-// and are left to k_stream2_keep.
+// and are carried over by keep() with the ghost shell.
 __global__ void k_stream_update_collide(const Dims d, const Arrays a, const double *__restrict__ src,
                                         double *__restrict__ dst, int collide_next)
 {
@@ -875,7 +875,7 @@ int phased_full_step(PhasedState *s, std::string &err)
 // the same results as n calls of phased_full_step: the first Collision on its
 // own, then per step the halo and boundary kernels on the post-collision f,
 // one k_stream_update_collide pass (with the next step's Collision, except
-// after the last step) and the ghost-shell carry-over of k_stream2_keep.
+// after the last step) and the carry-over of keep().
 int phased_steps(PhasedState *s, int n, std::string &err)
 {
     if (n <= 0) return IMEXLB_OK;
