@@ -150,8 +150,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lowp", action="store_true")
-    ap.add_argument("--placements", type=int, default=6,
-                    help="lattice allocations timed before the run, fastest kept (1: first only)")
+    ap.add_argument("--placements", type=int, default=0,
+                    help="lattice allocations timed before the run, fastest kept (1: first only; "
+                         "0: 8 / 8 / 12 for FP64 / FP32 / FP16, about 235 / 120 / 90 GB at 512^3)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -166,7 +167,8 @@ def main():
     cells = n ** 3
     log(f"[bench] TGV {n}^3 {a.storage} FP64: warmup {W}, timed {K}")
     launch = tuple(int(v) for v in a.launch.split(",")) if a.launch else None
-    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune, launch, a.placements)
+    trials = {dt: a.placements or {"f64": 8, "f32": 8, "f16": 12}[dt] for dt in ("f64", "f32", "f16")}
+    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune, launch, trials["f64"])
     glups64 = cells * K / (ms64 * 1e-3) / 1e9
     per_launch_ms = ms64 / K
     bytes_lup = info64["bytes_per_lup"]
@@ -181,7 +183,7 @@ def main():
     if not a.no_lowp:
         for dt in ("f32", "f16"):
             log(f"[bench] TGV {n}^3 {a.storage} {dt}")
-            g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re, not a.no_tune, None, a.placements)
+            g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re, not a.no_tune, None, trials[dt])
             gl = cells * K / (ms * 1e-3) / 1e9
             m = g.moments()
             err = vel_l2(m, m64)
