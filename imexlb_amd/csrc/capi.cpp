@@ -48,6 +48,10 @@ struct imexlb_handle {
     bool initialized = false;
     bool phase_collided = false;
     int64_t device_bytes = 0;
+    // HIP graph of GRAPH_STEPS moment-free steps, replayed by imexlb_step;
+    // valid for the lattices, start state and launch shape in gkey
+    hipGraphExec_t gexec = nullptr;
+    uint64_t gkey[8] = {};
     // phased engine
     PhasedState *ph = nullptr;
     std::string err;
@@ -236,6 +240,58 @@ static int set_device(imexlb_handle *h)
     return IMEXLB_OK;
 }
 
+// Steps per captured graph: even, so that the pull lattice index and the AA
+// layout parity are where they started when a replay ends.
+constexpr int GRAPH_STEPS = 16;
+
+static void graph_key(const imexlb_handle *h, uint64_t (&key)[8])
+{
+    key[0] = (uint64_t)(uintptr_t)h->lat[0];
+    key[1] = (uint64_t)(uintptr_t)h->lat[1];
+    key[2] = (uint64_t)h->cur;
+    key[3] = (uint64_t)(h->steps & 1);
+    key[4] = (uint64_t)h->block;
+    key[5] = (uint64_t)h->cpt;
+    key[6] = (uint64_t)h->xcd;
+    key[7] = (uint64_t)h->ntl;
+}
+
+// Capture GRAPH_STEPS moment-free fused steps from the current state (the
+// state itself is left as it was) unless a graph for this state is cached.
+// Only kernel launches are captured.
+static int ensure_graph(imexlb_handle *h)
+{
+    uint64_t key[8];
+    graph_key(h, key);
+    if (h->gexec && std::memcmp(key, h->gkey, sizeof key) == 0) return IMEXLB_OK;
+    if (h->gexec) {
+        hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    const int cur0 = h->cur;
+    const int64_t steps0 = h->steps, mom0 = h->mom_step;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = IMEXLB_OK;
+    for (int s = 0; s < GRAPH_STEPS && !rc; ++s) rc = fused_step(h, false);
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->cur = cur0;
+    h->steps = steps0;
+    h->mom_step = mom0;
+    if (rc || e != hipSuccess || !g) {
+        if (g) hipGraphDestroy(g);
+        return rc ? rc : fail(h, IMEXLB_EHIP, "step graph capture: %s", hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        h->gexec = nullptr;
+        return fail(h, IMEXLB_EHIP, "step graph instantiate: %s", hipGetErrorString(ei));
+    }
+    std::memcpy(h->gkey, key, sizeof key);
+    return IMEXLB_OK;
+}
+
 extern "C" {
 
 int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
@@ -393,6 +449,7 @@ int imexlb_destroy(imexlb_handle *h)
     if (h->result) hipFree(h->result);
     if (h->tgv_tab) hipFree(h->tgv_tab);
     if (h->ph) phased_destroy(h->ph);
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return IMEXLB_OK;
@@ -512,10 +569,24 @@ int imexlb_step(imexlb_handle *h, int nsteps, int moments_every)
         h->steps += nsteps;
         return IMEXLB_OK;
     }
-    for (int k = 0; k < nsteps; ++k) {
-        const int64_t idx = h->steps + 1;
-        const bool wm = moments_every > 0 && (idx % moments_every == 0 || k == nsteps - 1);
-        if ((rc = fused_step(h, wm))) return rc;
+    // steps that write no moments go GRAPH_STEPS at a time through a replayed
+    // graph (one launch instead of GRAPH_STEPS); the others one by one
+    auto writes_moments = [&](int64_t idx, int k) {
+        return moments_every > 0 && (idx % moments_every == 0 || k == nsteps - 1);
+    };
+    int k = 0;
+    while (k < nsteps) {
+        bool plain_run = nsteps - k >= GRAPH_STEPS;
+        for (int j = 0; plain_run && j < GRAPH_STEPS; ++j) plain_run = !writes_moments(h->steps + 1 + j, k + j);
+        if (plain_run) {
+            if ((rc = ensure_graph(h))) return rc;
+            HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+            h->steps += GRAPH_STEPS;  // an even count: cur and the AA parity are unchanged
+            k += GRAPH_STEPS;
+            continue;
+        }
+        if ((rc = fused_step(h, writes_moments(h->steps + 1, k)))) return rc;
+        ++k;
     }
     return IMEXLB_OK;
 }

@@ -123,7 +123,10 @@ int imexlb_update(imexlb_handle *h);   /* LBM::Update, tgv/src/lbm.cpp:288 */
 
 /* n whole steps (the six calls above, n times).  Moments are written on every
  * step whose index is a multiple of moments_every (> 0) and on the last step
- * of the call; moments_every = 0 writes none. */
+ * of the call; moments_every = 0 writes none.  Fused engine: runs of 16
+ * moment-free steps replay a captured HIP graph (same results).  Phased
+ * engine: per step, the halo and boundary kernels, then one pass of
+ * streaming, Update and the next Collision (same results as the six calls). */
 int imexlb_step(imexlb_handle *h, int nsteps, int moments_every);
 /* As imexlb_step, bracketed by HIP events on the handle's stream; *ms is the
  * device time of the n launches. */
