@@ -136,8 +136,12 @@ __global__ void k_init(const Dims d, const Arrays a)
     a.usr[c] = usr;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        a.f[(uint64_t)q * d.ncell + c] = feq_ref(q, p, u, v, w);
-        a.ft[(uint64_t)q * d.ncell + c] = 0.0;
+        const double fq = feq_ref(q, p, u, v, w);
+        a.f[(uint64_t)q * d.ncell + c] = fq;
+        // ft starts as a copy of f: after the swaps of Streaming either array
+        // is f, and ghost cells deeper than one layer, which the steps
+        // rewrite before reading or never write, must agree in both
+        a.ft[(uint64_t)q * d.ncell + c] = fq;
     }
 }
 
@@ -403,42 +407,49 @@ __global__ void k_stream2_keep(const Dims d, const double *__restrict__ f_old, d
     for (int q = 0; q < Q; ++q) f_new[(uint64_t)q * d.ncell + c] = f_old[(uint64_t)q * d.ncell + c];
 }
 
-// The ghost shell only (every case but the config channel, which also keeps
-// its solid cells): thread t maps to the t-th non-interior cell, the two z
-// slabs first, then the two y slabs and the two x slabs between them.
-__host__ __device__ __forceinline__ uint64_t shell_cells(const Dims &d)
+// The first ghost layer only (every case but the config channel, which also
+// keeps its solid cells).  Deeper ghost cells need no carrying over: every
+// step rewrites the ones it reads before reading them (the periodic halo, the
+// cavity's zeroed planes, the channel's inlet and outlet planes), and the
+// rest keep their initial values, which both arrays hold from Initialize.
+// Thread t maps to the t-th cell of the layer: the two z faces (with their
+// edges and corners) first, then the two y faces and the two x faces.
+__host__ __device__ __forceinline__ uint64_t layer_cells(const Dims &d)
 {
-    return d.ncell - (uint64_t)d.nx * d.ny * d.nz;
+    const uint64_t a = (uint64_t)d.nx + 2, b = (uint64_t)d.ny + 2;
+    return 2 * a * b + 2 * a * d.nz + 2 * (uint64_t)d.ny * d.nz;
 }
 
-__global__ void k_shell_keep(const Dims d, const double *__restrict__ f_old, double *__restrict__ f_new)
+__global__ void k_layer_keep(const Dims d, const double *__restrict__ f_old, double *__restrict__ f_new)
 {
     const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t0 >= shell_cells(d)) return;
+    if (t0 >= layer_cells(d)) return;
     uint32_t t = (uint32_t)t0;
-    const uint32_t lx = d.lx, ly = d.ly, nx = d.nx, ny = d.ny, nz = d.nz;
-    const uint32_t nzs = 2u * G * lx * ly, nys = 2u * G * lx * nz;
+    const uint32_t a = d.nx + 2, b = d.ny + 2, ny = d.ny, nz = d.nz;
     uint32_t i, j, k;
-    if (t < nzs) {
-        const uint32_t p = t / (lx * ly), r = t - p * (lx * ly);
-        k = p < (uint32_t)G ? p : nz + p;  // G + nz + (p - G)
-        j = r / lx;
-        i = r - j * lx;
-    } else if ((t -= nzs) < nys) {
-        const uint32_t p = t / (lx * nz), r = t - p * (lx * nz);
-        j = p < (uint32_t)G ? p : ny + p;
-        k = G + r / lx;
-        i = r - (k - G) * lx;
+    if (t < 2u * a * b) {
+        const uint32_t sd = t / (a * b), r = t - sd * (a * b);
+        k = sd ? G + nz : G - 1;
+        j = G - 1 + r / a;
+        i = G - 1 + (r - (j - (G - 1)) * a);
+    } else if ((t -= 2u * a * b) < 2u * a * nz) {
+        const uint32_t sd = t / (a * nz), r = t - sd * (a * nz);
+        j = sd ? G + ny : G - 1;
+        k = G + r / a;
+        i = G - 1 + (r - (k - G) * a);
     } else {
-        t -= nys;
-        const uint32_t p = t / (ny * nz), r = t - p * (ny * nz);
-        i = p < (uint32_t)G ? p : nx + p;
+        t -= 2u * a * nz;
+        const uint32_t sd = t / (ny * nz), r = t - sd * (ny * nz);
+        i = sd ? G + d.nx : G - 1;
         k = G + r / ny;
         j = G + (r - (k - G) * ny);
     }
     const uint64_t c = cidx(d, (int)i, (int)j, (int)k);
+    double v[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f_new[(uint64_t)q * d.ncell + c] = f_old[(uint64_t)q * d.ncell + c];
+    for (int q = 0; q < Q; ++q) v[q] = f_old[(uint64_t)q * d.ncell + c];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f_new[(uint64_t)q * d.ncell + c] = v[q];
 }
 
 __global__ void k_update(const Dims d, const Arrays a)
@@ -805,7 +816,7 @@ static void keep(PhasedState *s)
     if (d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG)
         hipLaunchKernelGGL(k_stream2_keep, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a.f, s->a.ft, s->a.usr);
     else
-        hipLaunchKernelGGL(k_shell_keep, grid_for(shell_cells(d)), dim3(256), 0, s->st, d, s->a.f, s->a.ft);
+        hipLaunchKernelGGL(k_layer_keep, grid_for(layer_cells(d)), dim3(256), 0, s->st, d, s->a.f, s->a.ft);
 }
 
 // Streaming's boundary kernels, in the reference's order, on s->a.f.
