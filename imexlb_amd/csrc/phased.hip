@@ -516,10 +516,14 @@ __global__ void k_stream_update_collide(const Dims d, const Arrays a, const doub
     decode(d, c, i, j, k);
     if (!interior(d, i, j, k)) return;
     const bool solid_kept = d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG && a.usr[c] == 0;
+    // as in the fused engine: lines no other wavefront reads (e_x = 0) are
+    // loaded non-temporally, and dst is written non-temporally
     double f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
-        f[q] = solid_kept ? src[(uint64_t)q * d.ncell + c] : src[fidx(d, q, i - EX[q], j - EY[q], k - EZ[q])];
+    for (int q = 0; q < Q; ++q) {
+        const double *p = src + (solid_kept ? (uint64_t)q * d.ncell + c : fidx(d, q, i - EX[q], j - EY[q], k - EZ[q]));
+        f[q] = EX[q] == 0 ? __builtin_nontemporal_load(p) : *p;
+    }
     double pm, um, vm, wm;
     moments_f64(f, pm, um, vm, wm);
     if (d.case_id == IMEXLB_CASE_SPHERE) {
@@ -555,7 +559,7 @@ __global__ void k_stream_update_collide(const Dims d, const Arrays a, const doub
         }
     }
 #pragma unroll
-    for (int q = 0; q < Q; ++q) dst[(uint64_t)q * d.ncell + c] = f[q];
+    for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(f[q], dst + (uint64_t)q * d.ncell + c);
 }
 
 __global__ void k_interior_io(const Dims d, double *f, double *buf, int nq, int to_buf)
