@@ -5,8 +5,8 @@ line.
     python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000]
     python -m imexlb_amd.runs sphere    [--steps 50000]
     python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256]
-    python -m imexlb_amd.runs timing    [--n 512] [--steps 20] [--dtype f16] [--storage aa]
-                                        [--launch B,C,X,N] [--placements 1]
+    python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
+                                        [--storage aa] [--launch B,C,X,N] [--placements 1]
 
 precision: TGV 512^3, U0 0.05, Re 1600, AA storage in FP64 / FP32 / FP16
     (shifted); kinetic energy against the analytic early-time decay
@@ -20,7 +20,7 @@ ensemble: 256 domains from seed 20261202 (TGV / cavity, edges 32..96,
     Re 50..1000, lid 0.05..0.1), FP32 shifted storage; persistent launch with
     graph replay vs persistent launches vs one launch per domain; per-domain
     velocity error against the same ensemble in FP64.
-timing: TGV n^3 (U0 0.05, Re 1600), one dtype and storage, a fixed launch
+timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, a fixed launch
     shape (or the autotuner's), `steps` timed steps after 4 warm-up steps;
     GLUPS and algorithmic GB/s.  The short run to profile one kernel with.
 """
@@ -207,21 +207,22 @@ def ensemble(a):
 
 def timing(a):
     n = a.n
-    tau = 0.05 * n / 1600.0 / (1.0 / 3.0) / 2 / PI_R
-    g = LBM(n, n, n, tau, u0=0.05, dtype=a.dtype, storage=a.storage)
+    shape = tuple(int(v) for v in a.shape.split(",")) if a.shape else (n, n, n)
+    tau = 0.05 * shape[0] / 1600.0 / (1.0 / 3.0) / 2 / PI_R
+    g = LBM(*shape, tau, u0=0.05, dtype=a.dtype, storage=a.storage)
     if a.launch:
         g.set_launch(*(int(v) for v in a.launch.split(",")))
         g.Initialize()
     else:
         g.autotune(None, steps=8)
-    out = {"workload": f"TGV {n}^3 Re 1600 {a.dtype} {a.storage}", "placement": None}
+    out = {"workload": f"TGV {'x'.join(map(str, shape))} Re 1600 {a.dtype} {a.storage}", "placement": None}
     if a.placements > 1:
         kept, ms_each = g.tune_placement(a.placements, steps=10)
         out["placement"] = {"kept": kept, "ms_per_step_each": ms_each}
     g.step(4, moments_every=0)
     ms = g.step_timed(a.steps, moments_every=0)
     info = g.info()
-    glups = n ** 3 * a.steps / (ms * 1e-3) / 1e9
+    glups = shape[0] * shape[1] * shape[2] * a.steps / (ms * 1e-3) / 1e9
     out.update({"steps": a.steps, "ms_per_step": ms / a.steps, "glups": glups,
                 "achieved_GBps": glups * info["bytes_per_lup"],
                 "launch": [info[k] for k in ("block", "cells_per_thread", "xcd_remap", "nt_loads")]})
@@ -239,6 +240,7 @@ def main(argv=None):
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--launch", default=None, help="block,cells_per_thread,xcd_order,nt_loads")
     ap.add_argument("--placements", type=int, default=1)
+    ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),
                 "timing": (512, 20)}
