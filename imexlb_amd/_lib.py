@@ -84,6 +84,7 @@ SIGNATURES = [
     ("imexlb_step", _i, [_vp, _i, _i]),
     ("imexlb_step_timed", _i, [_vp, _i, _i, _fp]),
     ("imexlb_get_moments", _i, [_vp, _dp, _dp, _dp, _dp]),
+    ("imexlb_get_macro", _i, [_vp, _dp, _dp, _dp, _dp]),
     ("imexlb_get_f", _i, [_vp, _dp]),
     ("imexlb_set_f", _i, [_vp, _dp]),
     ("imexlb_diagnostics", _i, [_vp, _dp, _dp]),

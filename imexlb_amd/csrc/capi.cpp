@@ -639,6 +639,15 @@ int imexlb_get_moments(imexlb_handle *h, double *p, double *u, double *v, double
     return IMEXLB_OK;
 }
 
+int imexlb_get_macro(imexlb_handle *h, double *rho, double *u, double *v, double *w)
+{
+    const int rc = imexlb_get_moments(h, rho, u, v, w);
+    if (rc || !rho) return rc;
+    const size_t n = (size_t)h->cfg.nx * h->cfg.ny * h->cfg.nz;
+    for (size_t k = 0; k < n; ++k) rho[k] = 3.0 * rho[k];
+    return IMEXLB_OK;
+}
+
 static int xfer(imexlb_handle *h, double *host, bool get)
 {
     const imexlb_config &c = h->cfg;

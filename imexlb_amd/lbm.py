@@ -121,6 +121,12 @@ class LBM:
         self._chk(lib().imexlb_get_moments(self.h, *[dptr(a) for a in out]))
         return out
 
+    def macro(self):
+        """rho = 3 p (= sum f), u, v, w of the last step that wrote moments."""
+        out = [np.empty(self.shape) for _ in range(4)]
+        self._chk(lib().imexlb_get_macro(self.h, *[dptr(a) for a in out]))
+        return out
+
     def get_f(self):
         out = np.empty((L.Q,) + self.shape)
         self._chk(lib().imexlb_get_f(self.h, dptr(out)))

@@ -135,6 +135,9 @@ int imexlb_step_timed(imexlb_handle *h, int nsteps, int moments_every, float *ms
 /* Moments p, u, v, w (the reference's p, ua, va, wa) of the last step that
  * wrote them, interior only, [z][y][x] with x fastest.  NULL skips a field. */
 int imexlb_get_moments(imexlb_handle *h, double *p, double *u, double *v, double *w);
+/* The same fields as density and velocity: rho = 3 p = sum f (the model is
+ * incompressible, tgv/src/lbm.cpp:307-325 sums p = sum f / 3), u, v, w. */
+int imexlb_get_macro(imexlb_handle *h, double *rho, double *u, double *v, double *w);
 /* Populations, interior, [i][z][y][x], FP64.  Fused engine: post-collision
  * populations of the last step (the engine's state; set_f restores it).
  * Phased engine: the reference's f array at this point of the step. */

@@ -289,3 +289,16 @@ def test_phased_step_equals_the_six_calls(case, mode):
         for x, y in zip(a.moments(), b.moments()):
             assert same(x, y)
         assert same(a.get_f(), b.get_f())
+
+
+def test_macro_is_three_p_and_u():
+    """imexlb_get_macro gives rho = 3 p and the velocity of imexlb_get_moments."""
+    from imexlb_amd import LBM
+    g = LBM(16, 12, 10, tgv_tau(16, 100), u0=0.05)
+    g.Initialize()
+    g.step(3)
+    p, u, v, w = g.moments()
+    rho, u2, v2, w2 = g.macro()
+    assert np.array_equal(rho, 3.0 * p)
+    for a, b in ((u, u2), (v, v2), (w, w2)):
+        assert np.array_equal(a, b)
