@@ -502,26 +502,42 @@ __global__ void k_update(const Dims d, const Arrays a)
 // f with its ghost shell already filled by the step's boundary kernels) into
 // `dst`.  The moments and their overrides are k_update's, the collision is
 // k_collide's, in the same operation order, so every value is the one the
-// phase-by-phase sequence produces; only the moments of ghost cells, which
-// nothing reads, are not written.  collide_next = 0 on the last step leaves
-// dst holding the streamed (pre-collision) populations, the state after
-// Update.  The channel's solid cells (config mode) keep their populations
-// and are carried over by keep() with the ghost shell.
+// phase-by-phase sequence produces.  last = 1 on the last step leaves dst
+// holding the streamed (pre-collision) populations, the state after Update,
+// and writes the moments of the interior; earlier steps write no moments, as
+// nothing reads them before the last Update (the collision takes them from
+// registers), and ghost moments are never read.
+//
+// The same pass carries over what the copy-back leaves alone: every cell of
+// the rows from the first ghost layer inwards (y and z) that is not an
+// interior fluid cell is copied from src to dst.  That covers the first ghost
+// layer and the channel's solid cells, and it also copies the x ghost columns
+// of those rows, so each row is written whole and no line of dst is left half
+// written.  Deeper ghost cells need no carrying over: every step rewrites the
+// ones it reads before reading them (the periodic halo, the cavity's zeroed
+// planes, the channel's inlet and outlet planes), and the rest keep their
+// initial values, which both arrays hold from Initialize; copying one of
+// those x columns therefore changes nothing.  Copy lanes take the same path
+// as the others, with their own cell as every source, so the wavefront does
+// not diverge.
 __global__ void k_stream_update_collide(const Dims d, const Arrays a, const double *__restrict__ src,
-                                        double *__restrict__ dst, int collide_next)
+                                        double *__restrict__ dst, int last)
 {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.ncell) return;
     int i, j, k;
     decode(d, c, i, j, k);
-    if (!interior(d, i, j, k)) return;
-    const bool solid_kept = d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG && a.usr[c] == 0;
+    if (j < G - 1 || j > G + d.ny || k < G - 1 || k > G + d.nz) return;
+    const bool in = interior(d, i, j, k);
+    const bool copy = !in || (d.case_id == IMEXLB_CASE_SPHERE && d.mode == IMEXLB_MODE_CONFIG && a.usr[c] == 0);
     // as in the fused engine: lines no other wavefront reads (e_x = 0) are
     // loaded non-temporally, and dst is written non-temporally
     double f[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const double *p = src + (solid_kept ? (uint64_t)q * d.ncell + c : fidx(d, q, i - EX[q], j - EY[q], k - EZ[q]));
+        const uint64_t own = (uint64_t)q * d.ncell + c;
+        const int64_t back = EX[q] + (int64_t)EY[q] * d.lx + (int64_t)EZ[q] * d.lx * d.ly;  // x - e_q
+        const double *p = src + (copy ? own : own - back);
         f[q] = EX[q] == 0 ? __builtin_nontemporal_load(p) : *p;
     }
     double pm, um, vm, wm;
@@ -546,16 +562,18 @@ __global__ void k_stream_update_collide(const Dims d, const Arrays a, const doub
             if (i == d.lx - G - 1) pm = 0.0;
         }
     }
-    a.ua[c] = um;
-    a.va[c] = vm;
-    a.wa[c] = wm;
-    a.p[c] = pm;
-    if (solid_kept) return;
-    if (collide_next) {
+    if (last && in) {
+        a.ua[c] = um;
+        a.va[c] = vm;
+        a.wa[c] = wm;
+        a.p[c] = pm;
+    }
+    if (!last) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const double feq = feq_ref(q, pm, um, vm, wm);
-            f[q] = f[q] - (f[q] - feq) / (d.tau0 + 0.5);
+            const double g = f[q] - (f[q] - feq) / (d.tau0 + 0.5);
+            f[q] = copy ? f[q] : g;
         }
     }
 #pragma unroll
@@ -890,7 +908,7 @@ int phased_full_step(PhasedState *s, std::string &err)
 // the same results as n calls of phased_full_step: the first Collision on its
 // own, then per step the halo and boundary kernels on the post-collision f,
 // one k_stream_update_collide pass (with the next step's Collision, except
-// after the last step) and the carry-over of keep().
+// after the last step), which also carries over what keep() would.
 int phased_steps(PhasedState *s, int n, std::string &err)
 {
     if (n <= 0) return IMEXLB_OK;
@@ -901,8 +919,7 @@ int phased_steps(PhasedState *s, int n, std::string &err)
         if ((rc = phased_unpack(s, err))) return rc;
         boundaries(s);
         hipLaunchKernelGGL(k_stream_update_collide, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a, s->a.f,
-                           s->a.ft, step < n ? 1 : 0);
-        keep(s);
+                           s->a.ft, step == n ? 1 : 0);
         PH_CHK(hipGetLastError());
         std::swap(s->a.f, s->a.ft);
     }
