@@ -112,6 +112,8 @@ SIGNATURES = [
     ("imexlb_ensemble_info", _i, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     ("imexlb_ensemble_set_launch", _i, [_vp, _i, _i, _i]),
     ("imexlb_ensemble_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
+    ("imexlb_ensemble_tune_placement", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float),
+                                            ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_ensemble_last_error", ctypes.c_char_p, [_vp]),
 ]
 

@@ -116,6 +116,9 @@ struct imexlb_ensemble {
     FusedParams *dp = nullptr;           // device domain table
     std::vector<uint2> tiles;
     uint2 *dtiles = nullptr;
+    void *pool = nullptr;                // every lattice, in one allocation
+    size_t pool_bytes = 0;
+    std::vector<size_t> lat_off;         // offset of each lattice in the pool
     std::vector<void *> lat;             // 2 per domain (second null for AA)
     std::vector<double *> mom;           // 4 per domain
     std::vector<double *> tabs;          // TGV init tables per domain
@@ -182,6 +185,22 @@ static int upload_tables(imexlb_ensemble *e)
     ECHK(e, hipMemcpy(e->dp, e->hp.data(), e->hp.size() * sizeof(FusedParams), hipMemcpyHostToDevice));
     drop_graph(e);
     return IMEXLB_OK;
+}
+
+// Point every lattice, and the domain table's src/dst, into `pool`.
+static int use_pool(imexlb_ensemble *e, void *pool)
+{
+    e->pool = pool;
+    const bool pull = e->storage == IMEXLB_PULL;
+    for (int d = 0; d < e->n; ++d) {
+        for (int k = 0; k < (pull ? 2 : 1); ++k) e->lat[2 * d + k] = (char *)pool + e->lat_off[2 * d + k];
+        for (int par = 0; par < 2; ++par) {
+            FusedParams &P = e->hp[2 * d + par];
+            P.src = pull ? e->lat[2 * d + par] : e->lat[2 * d];
+            P.dst = pull ? e->lat[2 * d + 1 - par] : e->lat[2 * d];
+        }
+    }
+    return upload_tables(e);
 }
 
 static int pattern_of(const imexlb_ensemble *e)
@@ -288,8 +307,7 @@ int imexlb_ensemble_destroy(imexlb_ensemble *e)
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
-    for (void *p : e->lat)
-        if (p) (void)hipFree(p);
+    if (e->pool) (void)hipFree(e->pool);
     for (double *p : e->mom)
         if (p) (void)hipFree(p);
     for (double *p : e->tabs)
@@ -332,18 +350,28 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
     const size_t elem = dtype == IMEXLB_F64 ? 8 : (dtype == IMEXLB_F32 ? 4 : 2);
     e->hp.resize(2 * (size_t)n);
     e->lat.assign(2 * (size_t)n, nullptr);
+    e->lat_off.assign(2 * (size_t)n, 0);
+    // one pool for all lattices, each on a 4 KiB boundary: the step rate
+    // depends on where a lattice's pages land, and a single allocation can be
+    // swapped for a faster one as a whole (imexlb_ensemble_tune_placement)
+    const int nl = storage == IMEXLB_PULL ? 2 : 1;
+    for (int d = 0; d < n; ++d) {
+        const uint64_t nc = (uint64_t)doms[d].nx * doms[d].ny * doms[d].nz;
+        for (int k = 0; k < nl; ++k) {
+            e->lat_off[2 * d + k] = e->pool_bytes;
+            e->pool_bytes += ((size_t)Q * nc * elem + 4095) / 4096 * 4096;
+        }
+    }
+    if (hipMalloc(&e->pool, e->pool_bytes) != hipSuccess) return bail(IMEXLB_ENOMEM);
+    e->device_bytes += (int64_t)e->pool_bytes;
+    for (int d = 0; d < n; ++d)
+        for (int k = 0; k < nl; ++k) e->lat[2 * d + k] = (char *)e->pool + e->lat_off[2 * d + k];
     e->mom.assign(4 * (size_t)n, nullptr);
     e->tabs.assign((size_t)n, nullptr);
     for (int d = 0; d < n; ++d) {
         const imexlb_domain &D = doms[d];
         const uint32_t nc = (uint32_t)((uint64_t)D.nx * D.ny * D.nz);
         e->cells += nc;
-        const size_t lb = (size_t)Q * nc * elem;
-        const int nl = storage == IMEXLB_PULL ? 2 : 1;
-        for (int k = 0; k < nl; ++k) {
-            if (hipMalloc(&e->lat[2 * d + k], lb) != hipSuccess) return bail(IMEXLB_ENOMEM);
-            e->device_bytes += lb;
-        }
         for (int k = 0; k < 4; ++k) {
             if (hipMalloc(&e->mom[4 * d + k], (size_t)nc * 8) != hipSuccess) return bail(IMEXLB_ENOMEM);
             e->device_bytes += (int64_t)nc * 8;
@@ -522,6 +550,56 @@ int imexlb_ensemble_set_launch(imexlb_ensemble *e, int block, int cpt, int grid)
     e->grid = grid;
     build_tiles(e);
     return upload_tables(e);
+}
+
+// As imexlb_tune_placement for one domain: time graph-replayed steps on the
+// current pool and on up to trials - 1 fresh ones (each held until the end,
+// so each has its own pages; 2 GiB stay free), keep the fastest, free the
+// others and reset the ensemble to its initial state.  The results do not
+// depend on the pool.
+int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, float *ms_per_step, int32_t *chosen)
+{
+    if (!e) return IMEXLB_EINVAL;
+    if (trials < 1 || trials > 16) return efail(e, IMEXLB_EINVAL, "placement trials %d must be in [1, 16]", trials);
+    ECHK(e, hipSetDevice(e->device));
+    if (steps < 2) steps = 2;
+    steps += steps & 1;
+    std::vector<void *> cand{e->pool};
+    std::vector<float> ms(trials, 0.0f);
+    auto time_current = [&](float &out) -> int {
+        int r = imexlb_ensemble_initialize(e);
+        if (!r) r = imexlb_ensemble_step(e, 2, IMEXLB_ENS_GRAPH, 0);
+        if (!r) r = imexlb_ensemble_step_timed(e, steps, IMEXLB_ENS_GRAPH, 0, &out);
+        if (!r) out /= (float)steps;
+        return r;
+    };
+    int rc = time_current(ms[0]);
+    for (int t = 1; !rc && t < trials; ++t) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < e->pool_bytes + (2ull << 30)) break;
+        void *p = nullptr;
+        if (hipMalloc(&p, e->pool_bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            break;
+        }
+        cand.push_back(p);
+        rc = use_pool(e, p);
+        if (!rc) rc = time_current(ms[t]);
+    }
+    int best = 0;
+    if (!rc)
+        for (int t = 1; t < (int)cand.size(); ++t)
+            if (ms[t] < ms[best]) best = t;
+    (void)hipStreamSynchronize(e->stream);
+    for (int t = 0; t < (int)cand.size(); ++t)
+        if (t != best) (void)hipFree(cand[t]);
+    int rc2 = use_pool(e, cand[best]);
+    if (!rc2) rc2 = imexlb_ensemble_initialize(e);
+    if (rc) return rc;
+    if (ms_per_step)
+        for (int t = 0; t < trials; ++t) ms_per_step[t] = ms[t];
+    if (chosen) *chosen = best;
+    return rc2;
 }
 
 const char *imexlb_ensemble_last_error(const imexlb_ensemble *e) { return e ? e->err.c_str() : ""; }

@@ -288,6 +288,16 @@ class Ensemble:
                                                  int(steps), ch))
         return tuple(ch)
 
+    def tune_placement(self, trials=6, steps=6):
+        """Time graph-replayed steps on up to `trials` allocations of the
+        ensemble's lattices and keep the fastest; the ensemble is reset to
+        its initial state.  Returns (index kept, [ms per step of each
+        candidate, 0 if none])."""
+        ms = (ctypes.c_float * int(trials))()
+        ch = ctypes.c_int32()
+        self._chk(lib().imexlb_ensemble_tune_placement(self.h, int(trials), int(steps), ms, ctypes.byref(ch)))
+        return ch.value, list(ms)
+
     def info(self):
         out = (ctypes.c_int64 * 7)()
         self._chk(lib().imexlb_ensemble_info(self.h, out))

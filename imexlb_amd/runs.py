@@ -4,7 +4,7 @@ line.
     python -m imexlb_amd.runs precision [--n 512] [--steps 5000]
     python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000]
     python -m imexlb_amd.runs sphere    [--steps 50000]
-    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256]
+    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements 6]
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
 
@@ -167,6 +167,12 @@ def ensemble(a):
     e = Ensemble(doms, dtype="f32", storage="pull")
     e.Initialize()
     out["autotune"] = e.autotune(None, steps=4)
+    trials = 6 if a.placements is None else a.placements
+    if trials > 1:
+        kept, ms_each = e.tune_placement(trials, steps=6)
+        out["placement"] = {"trials": trials, "kept": kept,
+                            "glups_each": [round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
+        log(f"[ensemble] placement: {out['placement']}")
     res = {}
     for mode in ("graph", "persistent", "per_domain"):
         e.Initialize()
@@ -216,7 +222,7 @@ def timing(a):
     else:
         g.autotune(None, steps=8)
     out = {"workload": f"TGV {'x'.join(map(str, shape))} Re 1600 {a.dtype} {a.storage}", "placement": None}
-    if a.placements > 1:
+    if (a.placements or 1) > 1:
         kept, ms_each = g.tune_placement(a.placements, steps=10)
         out["placement"] = {"kept": kept, "ms_per_step_each": ms_each}
     g.step(4, moments_every=0)
@@ -239,7 +245,8 @@ def main(argv=None):
     ap.add_argument("--dtype", default="f16", choices=["f64", "f32", "f16"])
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--launch", default=None, help="block,cells_per_thread,xcd_order,nt_loads")
-    ap.add_argument("--placements", type=int, default=1)
+    ap.add_argument("--placements", type=int, default=None,
+                    help="lattice allocations to try (default 1 for timing, 6 for ensemble)")
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),

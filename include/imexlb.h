@@ -224,6 +224,11 @@ int imexlb_ensemble_set_launch(imexlb_ensemble *e, int block, int cells_per_thre
 /* As imexlb_autotune, over block x cells per thread x persistent grid size,
  * timed with graph replay; chosen[3] = block, cells per thread, grid. */
 int imexlb_ensemble_autotune(imexlb_ensemble *e, const char *cache_path, int steps, int32_t *chosen);
+/* As imexlb_tune_placement, for the ensemble's lattices, which share one
+ * allocation: up to trials - 1 fresh allocations of the whole set are timed
+ * with graph-replayed steps, the fastest is kept and the ensemble is reset
+ * to its initial state.  trials in [1, 16]. */
+int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, float *ms_per_step, int32_t *chosen);
 const char *imexlb_ensemble_last_error(const imexlb_ensemble *e);
 
 /* ---- host-side pieces of the reference, no device use ----------------- */

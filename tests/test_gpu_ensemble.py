@@ -102,3 +102,25 @@ def test_single_autotune_keeps_results(tmp_path):
     b.step(11)
     for x, y in zip(a.moments(), b.moments()):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_ensemble_placement_keeps_results(storage):
+    """imexlb_ensemble_tune_placement swaps the lattices' allocation and
+    resets the ensemble; the members then step bit for bit as before."""
+    from imexlb_amd import Ensemble
+    doms = small_domains()
+    e = Ensemble(doms, dtype="f32", storage=storage)
+    e.Initialize()
+    e.step(9, mode="graph")
+    ref = [e.moments(k) for k in range(len(doms))]
+    kept, ms = e.tune_placement(trials=3, steps=4)
+    print(f"[ensemble placement] {storage}: kept {kept} of ms/step {ms}", flush=True)
+    assert 0 <= kept < 3 and ms[kept] > 0 and ms[kept] == min(m for m in ms if m > 0)
+    assert e.info()["steps"] == 0
+    for mode in ("graph", "per_domain"):
+        e.Initialize()
+        e.step(9, mode=mode)
+        for k in range(len(doms)):
+            for a, b in zip(e.moments(k), ref[k]):
+                assert np.array_equal(a, b), (storage, mode, k)
