@@ -18,10 +18,6 @@
 
 using namespace imx;
 
-#ifndef IMEXLB_DEFAULT_PAD
-#define IMEXLB_DEFAULT_PAD 2304
-#endif
-
 struct imexlb_handle {
     imexlb_config cfg{};
     hipStream_t stream = nullptr;

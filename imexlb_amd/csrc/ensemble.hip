@@ -354,12 +354,16 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
     // one pool for all lattices, each on a 4 KiB boundary: the step rate
     // depends on where a lattice's pages land, and a single allocation can be
     // swapped for a faster one as a whole (imexlb_ensemble_tune_placement)
+    // population stride as in the single-domain engine: the cell count
+    // rounded up to 256 bytes plus the pad that staggers the 27 streams
     const int nl = storage == IMEXLB_PULL ? 2 : 1;
+    std::vector<uint64_t> pstride(n);
     for (int d = 0; d < n; ++d) {
-        const uint64_t nc = (uint64_t)doms[d].nx * doms[d].ny * doms[d].nz;
+        const uint64_t nc = (uint64_t)doms[d].nx * doms[d].ny * doms[d].nz, per = 256 / elem;
+        pstride[d] = (nc + per - 1) / per * per + (uint64_t)IMEXLB_DEFAULT_PAD / elem;
         for (int k = 0; k < nl; ++k) {
             e->lat_off[2 * d + k] = e->pool_bytes;
-            e->pool_bytes += ((size_t)Q * nc * elem + 4095) / 4096 * 4096;
+            e->pool_bytes += ((size_t)Q * pstride[d] * elem + 4095) / 4096 * 4096;
         }
     }
     if (hipMalloc(&e->pool, e->pool_bytes) != hipSuccess) return bail(IMEXLB_ENOMEM);
@@ -383,7 +387,7 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
             P.ny = D.ny;
             P.nz = D.nz;
             P.ncell = nc;
-            P.pstride = nc;
+            P.pstride = pstride[d];
             if (storage == IMEXLB_PULL) {
                 P.src = e->lat[2 * d + par];
                 P.dst = e->lat[2 * d + 1 - par];

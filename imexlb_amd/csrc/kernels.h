@@ -7,6 +7,13 @@
 
 #include "lattice.h"
 
+// Bytes added to each population stride of a fused-engine lattice (after
+// rounding the cell count up to 256 bytes): 9 x 256 keeps the 27 streams off a
+// common power-of-two alignment.
+#ifndef IMEXLB_DEFAULT_PAD
+#define IMEXLB_DEFAULT_PAD 2304
+#endif
+
 namespace imx {
 
 enum Bc : int { BC_PERIODIC = 0, BC_CAVITY = 1, BC_CHANNEL = 2 };

@@ -4,7 +4,7 @@ line.
     python -m imexlb_amd.runs precision [--n 512] [--steps 5000]
     python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000]
     python -m imexlb_amd.runs sphere    [--steps 50000]
-    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements 6]
+    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements 8]
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
 
@@ -167,7 +167,7 @@ def ensemble(a):
     e = Ensemble(doms, dtype="f32", storage="pull")
     e.Initialize()
     out["autotune"] = e.autotune(None, steps=4)
-    trials = 6 if a.placements is None else a.placements
+    trials = 8 if a.placements is None else a.placements
     if trials > 1:
         kept, ms_each = e.tune_placement(trials, steps=6)
         out["placement"] = {"trials": trials, "kept": kept,
@@ -246,7 +246,7 @@ def main(argv=None):
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--launch", default=None, help="block,cells_per_thread,xcd_order,nt_loads")
     ap.add_argument("--placements", type=int, default=None,
-                    help="lattice allocations to try (default 1 for timing, 6 for ensemble)")
+                    help="lattice allocations to try (default 1 for timing, 8 for ensemble)")
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),
