@@ -126,7 +126,9 @@ int imexlb_update(imexlb_handle *h);   /* LBM::Update, tgv/src/lbm.cpp:288 */
  * of the call; moments_every = 0 writes none.  Fused engine: runs of 16
  * moment-free steps replay a captured HIP graph (same results).  Phased
  * engine: per step, the halo and boundary kernels, then one pass of
- * streaming, Update and the next Collision (same results as the six calls). */
+ * streaming, Update and the next Collision (same results as the six calls);
+ * it writes the moments of the last step whatever moments_every is, as the
+ * reference's Update leaves them after every step. */
 int imexlb_step(imexlb_handle *h, int nsteps, int moments_every);
 /* As imexlb_step, bracketed by HIP events on the handle's stream; *ms is the
  * device time of the n launches. */
