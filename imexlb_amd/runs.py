@@ -7,6 +7,7 @@ line.
     python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements 8]
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
+    python -m imexlb_amd.runs phased    [--n 256] [--steps 20]
 
 precision: TGV 512^3, U0 0.05, Re 1600, AA storage in FP64 / FP32 / FP16
     (shifted); kinetic energy against the analytic early-time decay
@@ -23,6 +24,10 @@ ensemble: 256 domains from seed 20261202 (TGV / cavity, edges 32..96,
 timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, a fixed launch
     shape (or the autotuner's), `steps` timed steps after 4 warm-up steps;
     GLUPS and algorithmic GB/s.  The short run to profile one kernel with.
+phased: the reference's own boundary code (faithful mode) on the phased
+    engine's multi-step path: the cavity at n^3 (lid 0.1, tau0 0.096) and the
+    sphere input of sphere/input.in (512 x 128 x 128, Re 100, u0 0.05); GLUPS
+    over `steps` timed steps after 3 warm-up steps.
 """
 import argparse
 import json
@@ -236,9 +241,28 @@ def timing(a):
     return out
 
 
+def phased(a):
+    n = a.n
+    r = (512 - 1) / 64.0
+    cases = {f"cavity {n}^3": dict(shape=(n, n, n), tau=0.096, u0=0.1, case="cavity"),
+             "sphere 512x128x128 Re 100": dict(shape=(512, 128, 128), tau=0.05 * 2 * r / 100.0 / (1.0 / 3.0),
+                                               u0=0.05, case="sphere")}
+    out = {"workload": "faithful mode, phased engine", "runs": {}}
+    for name, c in cases.items():
+        g = LBM(*c["shape"], c["tau"], u0=c["u0"], case=c["case"], mode="faithful", engine="phased")
+        g.Initialize()
+        g.step(3)
+        ms = g.step_timed(a.steps)
+        cells = c["shape"][0] * c["shape"][1] * c["shape"][2]
+        out["runs"][name] = {"glups": cells * a.steps / (ms * 1e-3) / 1e9, "ms_per_step": ms / a.steps}
+        log(f"[phased] {name}: {out['runs'][name]['glups']:.2f} GLUPS")
+        g.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["precision", "cavity", "sphere", "ensemble", "timing"])
+    ap.add_argument("workload", choices=["precision", "cavity", "sphere", "ensemble", "timing", "phased"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--domains", type=int, default=256)
@@ -250,11 +274,12 @@ def main(argv=None):
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),
-                "timing": (512, 20)}
+                "timing": (512, 20), "phased": (256, 20)}
     dn, ds = defaults[a.workload]
     a.n = a.n or dn
     a.steps = a.steps or ds
-    fn = {"precision": precision, "cavity": cavity, "sphere": sphere, "ensemble": ensemble, "timing": timing}[a.workload]
+    fn = {"precision": precision, "cavity": cavity, "sphere": sphere, "ensemble": ensemble, "timing": timing,
+          "phased": phased}[a.workload]
     out = fn(a)
     out["copy_GBps"] = copy_bandwidth(1 << 30, reps=5)[0]
     print(json.dumps(out), flush=True)
