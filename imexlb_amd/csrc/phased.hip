@@ -188,7 +188,17 @@ __global__ void k_periodic(const Dims d, const Arrays a, int px, int py, int pz)
 }
 
 // the planes y = le1, z = le2, x = ls0 - 1 and x = le0, one thread per cell
-// of each plane (cells on two planes are zeroed twice)
+// of each plane (cells on two planes are zeroed twice).
+// In the two x planes only the edge cells (y or z on the first ghost layer)
+// are written; the rest of those planes already hold what the zero fill
+// would leave there.  Their cells outside the first ghost layer are never
+// written by any kernel and stay at the zeros of Initialize (f = 0 in the
+// faithful cavity).  Inside it, the populations pointing into the domain are
+// rewritten by bcl / bcr, the first two boundary kernels, before anything
+// reads them (neither reads the other's plane: nx >= 2, checked at create),
+// and the other 18 are written by no kernel: the y and z walls write only
+// their own planes, and the carry-over copies values unchanged.  This saves
+// a strided write of every population of every cell of both planes.
 __global__ void k_cavity_zero_planes(const Dims d, const Arrays a)
 {
     const uint32_t nyz = (uint32_t)d.ly * d.lz, nxz = (uint32_t)d.lx * d.lz, nxy = (uint32_t)d.lx * d.ly;
@@ -199,6 +209,9 @@ __global__ void k_cavity_zero_planes(const Dims d, const Arrays a)
         i = p == 0 ? G - 1 : G + d.nx;
         k = (int)(r / d.ly);
         j = (int)(r - (uint32_t)k * d.ly);
+        const bool ring_j = j >= G - 1 && j <= G + d.ny, ring_k = k >= G - 1 && k <= G + d.nz;
+        const bool edge = j == G - 1 || j == G + d.ny || k == G - 1 || k == G + d.nz;
+        if (!(ring_j && ring_k && edge)) return;
     } else if ((t -= 2u * nyz) < nxz) {
         j = G + d.ny;
         k = (int)(t / d.lx);
