@@ -3,7 +3,7 @@ line.
 
     python -m imexlb_amd.runs precision [--n 512] [--steps 5000]
     python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000]
-    python -m imexlb_amd.runs sphere    [--steps 50000]
+    python -m imexlb_amd.runs sphere    [--steps 50000] [--placements 8]
     python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements 8]
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
@@ -138,9 +138,16 @@ def sphere(a):
     sph = (128.0, 128.0, 128.0, d / 2)
     runs = {dt: LBM(nx, ny, nz, tau, u0=u_in, case="sphere", mode="config", dtype=dt, storage="pull",
                     sphere=sph) for dt in ("f64", "f32", "f16")}
-    for g in runs.values():
+    out = {"workload": f"sphere channel {nx}x{ny}x{nz} D {d:g} Re {re:g} u_in {u_in}", "tau0": tau, "drag": [],
+           "placement": {}}
+    trials = 8 if a.placements is None else a.placements
+    for dt, g in runs.items():
         g.autotune(None, steps=4)
-    out = {"workload": f"sphere channel {nx}x{ny}x{nz} D {d:g} Re {re:g} u_in {u_in}", "tau0": tau, "drag": []}
+        if trials > 1:
+            kept, ms_each = g.tune_placement(trials, steps=10)
+            out["placement"][dt] = {"kept": kept, "glups_each": [
+                round(nx * ny * nz / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
+            log(f"[sphere] {dt} placement: {out['placement'][dt]}")
     every = 1000
     t_dev = {dt: 0.0 for dt in runs}
     for s in range(every, a.steps + 1, every):
@@ -270,7 +277,7 @@ def main(argv=None):
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--launch", default=None, help="block,cells_per_thread,xcd_order,nt_loads")
     ap.add_argument("--placements", type=int, default=None,
-                    help="lattice allocations to try (default 1 for timing, 8 for ensemble)")
+                    help="lattice allocations to try (default 1 for timing, 8 for sphere and ensemble)")
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),
