@@ -2,7 +2,7 @@
 line.
 
     python -m imexlb_amd.runs precision [--n 512] [--steps 5000]
-    python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000]
+    python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000] [--placements 8]
     python -m imexlb_amd.runs sphere    [--steps 50000] [--placements 8]
     python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements 8]
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
@@ -104,9 +104,15 @@ def cavity(a):
         "fused_aa": LBM(n, n, n, tau, u0=0.0, case="cavity", mode="config", storage="aa", lid_u=lid),
         "phased": LBM(n, n, n, tau, u0=0.0, case="cavity", mode="config", engine="phased", lid_u=lid),
     }
-    for g in engines.values():
+    out = {"workload": f"cavity {n}^3 Re {re:g} lid {lid}", "tau0": tau, "checks": [], "placement": {}}
+    trials = 8 if a.placements is None else a.placements
+    for name, g in engines.items():
         g.Initialize()
-    out = {"workload": f"cavity {n}^3 Re {re:g} lid {lid}", "tau0": tau, "checks": []}
+        if trials > 1 and name != "phased":  # the phased engine keeps its own layout
+            kept, ms_each = g.tune_placement(trials, steps=20)
+            out["placement"][name] = {"kept": kept, "glups_each": [
+                round(n ** 3 / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
+            log(f"[cavity] {name} placement: {out['placement'][name]}")
     t = {k: 0.0 for k in engines}
     every = 1000
     for s in range(every, steps + 1, every):
@@ -277,7 +283,7 @@ def main(argv=None):
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--launch", default=None, help="block,cells_per_thread,xcd_order,nt_loads")
     ap.add_argument("--placements", type=int, default=None,
-                    help="lattice allocations to try (default 1 for timing, 8 for sphere and ensemble)")
+                    help="lattice allocations to try (default 1 for timing, 8 for the others)")
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),
