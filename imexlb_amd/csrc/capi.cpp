@@ -706,21 +706,26 @@ int imexlb_set_f(imexlb_handle *h, const double *f)
 }
 
 static const char CK_MAGIC[8] = {'I', 'M', 'X', 'L', 'B', 'C', 'K', '1'};
+static const char CK_MAGIC_PHASED[8] = {'I', 'M', 'X', 'L', 'B', 'P', 'H', '1'};
 
 int imexlb_checkpoint_write(imexlb_handle *h, const char *path)
 {
     if (!h || !path) return IMEXLB_EINVAL;
-    if (h->ph) return fail(h, IMEXLB_EUNSUPPORTED, "checkpoints hold the fused engine's state");
     if (!h->initialized) return fail(h, IMEXLB_ESTATE, "checkpoint before Initialize");
-    const size_t n = (size_t)Q * h->ncell;
+    int rc = set_device(h);
+    if (rc) return rc;
+    // fused: the post-collision populations of the interior; phased: the
+    // engine's whole state, ghost cells and moments included
+    const size_t n = h->ph ? (size_t)phased_state_doubles(h->ph) : (size_t)Q * h->ncell;
     std::vector<double> f(n);
-    int rc = imexlb_get_f(h, f.data());
+    rc = h->ph ? phased_get_state(h->ph, f.data(), h->err) : imexlb_get_f(h, f.data());
     if (rc) return rc;
     FILE *fp = std::fopen(path, "wb");
     if (!fp) return fail(h, IMEXLB_EIO, "cannot open %s", path);
     const int64_t steps = h->steps;
-    bool ok = std::fwrite(CK_MAGIC, 1, 8, fp) == 8 && std::fwrite(&h->cfg, sizeof h->cfg, 1, fp) == 1 &&
-              std::fwrite(&steps, 8, 1, fp) == 1 && std::fwrite(f.data(), 8, n, fp) == n;
+    bool ok = std::fwrite(h->ph ? CK_MAGIC_PHASED : CK_MAGIC, 1, 8, fp) == 8 &&
+              std::fwrite(&h->cfg, sizeof h->cfg, 1, fp) == 1 && std::fwrite(&steps, 8, 1, fp) == 1 &&
+              std::fwrite(f.data(), 8, n, fp) == n;
     if (std::fclose(fp) != 0) ok = false;
     return ok ? IMEXLB_OK : fail(h, IMEXLB_EIO, "short write to %s", path);
 }
@@ -728,7 +733,6 @@ int imexlb_checkpoint_write(imexlb_handle *h, const char *path)
 int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
 {
     if (!h || !path) return IMEXLB_EINVAL;
-    if (h->ph) return fail(h, IMEXLB_EUNSUPPORTED, "checkpoints hold the fused engine's state");
     if (!h->initialized) {
         int rc = imexlb_initialize(h);
         if (rc) return rc;
@@ -738,10 +742,17 @@ int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
     char magic[8];
     imexlb_config c;
     int64_t steps = 0;
-    const size_t n = (size_t)Q * h->ncell;
+    const size_t n = h->ph ? (size_t)phased_state_doubles(h->ph) : (size_t)Q * h->ncell;
     std::vector<double> f(n);
-    bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, CK_MAGIC, 8) == 0 &&
-              std::fread(&c, sizeof c, 1, fp) == 1 && std::fread(&steps, 8, 1, fp) == 1;
+    bool ok = std::fread(magic, 1, 8, fp) == 8 && std::fread(&c, sizeof c, 1, fp) == 1 &&
+              std::fread(&steps, 8, 1, fp) == 1;
+    const bool phased_file = ok && std::memcmp(magic, CK_MAGIC_PHASED, 8) == 0;
+    if (ok && !phased_file && std::memcmp(magic, CK_MAGIC, 8) != 0) ok = false;
+    if (ok && phased_file != (h->ph != nullptr)) {
+        std::fclose(fp);
+        return fail(h, IMEXLB_EINVAL, "checkpoint %s holds the %s engine's state", path,
+                    phased_file ? "phased" : "fused");
+    }
     if (ok && (c.case_id != h->cfg.case_id || c.mode != h->cfg.mode || c.nx != h->cfg.nx || c.ny != h->cfg.ny ||
                c.nz != h->cfg.nz)) {
         std::fclose(fp);
@@ -750,7 +761,13 @@ int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
     ok = ok && steps >= 0 && std::fread(f.data(), 8, n, fp) == n;
     std::fclose(fp);
     if (!ok) return fail(h, IMEXLB_EIO, "%s is not a complete checkpoint", path);
+    int rc = set_device(h);
+    if (rc) return rc;
     h->steps = steps;  // fixes the AA layout parity before the populations go in
+    if (h->ph) {
+        h->phase_collided = false;
+        return phased_set_state(h->ph, f.data(), h->err);
+    }
     return imexlb_set_f(h, f.data());
 }
 

@@ -32,5 +32,9 @@ int phased_get_moments(PhasedState *s, double *p, double *u, double *v, double *
 int phased_get_f(PhasedState *s, double *f, std::string &err);
 int phased_set_f(PhasedState *s, const double *f, std::string &err);
 int phased_diagnostics(PhasedState *s, double *ke, double *mass, std::string &err);
+// whole state for restart files: f, p, ua, va, wa over the ghosted box
+int64_t phased_state_doubles(const PhasedState *s);
+int phased_get_state(PhasedState *s, double *host, std::string &err);
+int phased_set_state(PhasedState *s, const double *host, std::string &err);
 
 }  // namespace imx

@@ -983,6 +983,36 @@ int phased_set_f(PhasedState *s, const double *f, std::string &err)
     return interior_copy(s, s->a.f, const_cast<double *>(f), Q, false, err);
 }
 
+// The whole state between steps, ghost cells included (the faithful wall
+// rules read ghost values written in earlier steps): f, then p, ua, va, wa,
+// each over the ghosted box.
+int64_t phased_state_doubles(const PhasedState *s) { return (int64_t)(Q + 4) * (int64_t)s->d.ncell; }
+
+int phased_get_state(PhasedState *s, double *host, std::string &err)
+{
+    const size_t n = s->d.ncell;
+    PH_CHK(hipMemcpyAsync(host, s->a.f, Q * n * 8, hipMemcpyDeviceToHost, s->st));
+    double *mom[4] = {s->a.p, s->a.ua, s->a.va, s->a.wa};
+    for (int k = 0; k < 4; ++k)
+        PH_CHK(hipMemcpyAsync(host + (Q + k) * n, mom[k], n * 8, hipMemcpyDeviceToHost, s->st));
+    PH_CHK(hipStreamSynchronize(s->st));
+    return IMEXLB_OK;
+}
+
+// f and ft both take the saved f: ghost cells deeper than the carried-over
+// layer must agree in the two arrays, as after Initialize.
+int phased_set_state(PhasedState *s, const double *host, std::string &err)
+{
+    const size_t n = s->d.ncell;
+    PH_CHK(hipMemcpyAsync(s->a.f, host, Q * n * 8, hipMemcpyHostToDevice, s->st));
+    PH_CHK(hipMemcpyAsync(s->a.ft, s->a.f, Q * n * 8, hipMemcpyDeviceToDevice, s->st));
+    double *mom[4] = {s->a.p, s->a.ua, s->a.va, s->a.wa};
+    for (int k = 0; k < 4; ++k)
+        PH_CHK(hipMemcpyAsync(mom[k], host + (Q + k) * n, n * 8, hipMemcpyHostToDevice, s->st));
+    PH_CHK(hipStreamSynchronize(s->st));
+    return IMEXLB_OK;
+}
+
 int phased_diagnostics(PhasedState *s, double *ke, double *mass, std::string &err)
 {
     hipLaunchKernelGGL(k_diag1, dim3(s->nparts), dim3(256), 0, s->st, s->d, s->a, s->partials);

@@ -147,10 +147,14 @@ int imexlb_get_f(imexlb_handle *h, double *f);
 int imexlb_set_f(imexlb_handle *h, const double *f);
 
 /* Restart file (no counterpart in the reference, which writes only the
- * macroscopic fields): "IMXLBCK1", the creating configuration, the step
- * count, then the post-collision populations [i][z][y][x] as FP64.  Reading
- * needs a handle of the same case, mode and extents (storage type and
- * precision may differ) and resumes at the saved step.  Fused engine. */
+ * macroscopic fields).  Fused engine: "IMXLBCK1", the creating
+ * configuration, the step count, then the post-collision populations
+ * [i][z][y][x] as FP64; a handle of the same case, mode and extents reads it
+ * (storage type and precision may differ).  Phased engine: "IMXLBPH1", the
+ * configuration, the step count, then the engine's whole state between steps
+ * as FP64 -- f, p, ua, va, wa over the ghosted box, since the faithful wall
+ * rules read ghost values written in earlier steps; a phased handle of the
+ * same case, mode and extents reads it.  Either resumes at the saved step. */
 int imexlb_checkpoint_write(imexlb_handle *h, const char *path);
 int imexlb_checkpoint_read(imexlb_handle *h, const char *path);
 

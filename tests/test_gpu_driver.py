@@ -66,6 +66,19 @@ def test_driver_restart_continues_the_run(tmp_path, storage):
     assert (ke, mass) == (ke_full, mass_full)
 
 
+def test_driver_restart_faithful_cavity(tmp_path):
+    """The reference's own cavity (faithful mode, phased engine): 12 steps, a
+    restart file with the ghost cells, then 9 more end where 21 steps end."""
+    text = "1.0  0.1  10\n0.1  21  0\n20 16 12\n"
+    common = ["--case", "cavity", "--mode", "faithful", "--honour-input"]
+    ke_full, mass_full, _ = run(tmp_path, text, common)
+    ck = str(tmp_path / "cav.ck")
+    run(tmp_path, text, common + ["--steps", "12", "--checkpoint", ck])
+    ke, mass, out = run(tmp_path, text, common + ["--steps", "9", "--restart", ck])
+    assert "Restarted from" in out
+    assert (ke, mass) == (ke_full, mass_full)
+
+
 def test_driver_output_files(tmp_path):
     """--output writes output0.plt (the initial fields) and one file every
     `inter` steps, as the cavity driver does (lid_driven_cavity/main.cpp:26,
