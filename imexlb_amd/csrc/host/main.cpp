@@ -102,8 +102,14 @@ int main(int argc, char **argv)
         }
         CommHelper comm(1, 0, opt.case_id == IMEXLB_CASE_TGV, s1.sx, s1.sy, s1.sz);
         LBM l1(comm, s1.sx, s1.sy, s1.sz, s1.tau, s1.rho0, s1.u0, opt);
-        if (!tune.empty()) l1.autotune(tune.c_str());
-        if (place > 1) std::cout << "Lattice placement kept: " << l1.tune_placement(place) << " of " << place << std::endl;
+        if (opt.engine == IMEXLB_ENGINE_PHASED && (!tune.empty() || place > 1)) {
+            std::cerr << "note: --tune and --place apply to the fused engine; the phased engine keeps the "
+                         "reference's layout" << std::endl;
+        } else {
+            if (!tune.empty()) l1.autotune(tune.c_str());
+            if (place > 1)
+                std::cout << "Lattice placement kept: " << l1.tune_placement(place) << " of " << place << std::endl;
+        }
         l1.Initialize();
         l1.setup_subdomain();
         std::cout << "Currently using HIP device: " << opt.device << std::endl;
