@@ -351,11 +351,11 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
     e->hp.resize(2 * (size_t)n);
     e->lat.assign(2 * (size_t)n, nullptr);
     e->lat_off.assign(2 * (size_t)n, 0);
-    // one pool for all lattices, each on a 4 KiB boundary: the step rate
+    // One pool for all lattices, each on a 4 KiB boundary: the step rate
     // depends on where a lattice's pages land, and a single allocation can be
-    // swapped for a faster one as a whole (imexlb_ensemble_tune_placement)
-    // population stride as in the single-domain engine: the cell count
-    // rounded up to 256 bytes plus the pad that staggers the 27 streams
+    // swapped for a faster one as a whole (imexlb_ensemble_tune_placement).
+    // Population stride as in the single-domain engine: the cell count
+    // rounded up to 256 bytes plus the pad that staggers the 27 streams.
     const int nl = storage == IMEXLB_PULL ? 2 : 1;
     std::vector<uint64_t> pstride(n);
     for (int d = 0; d < n; ++d) {
