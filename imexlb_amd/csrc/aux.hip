@@ -321,4 +321,111 @@ hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// The faithful cavity on the fused engine: Streaming's boundary phase on the
+// ring (the first ghost layer), in the reference's order, before the fused
+// pass reads it.  Interior values come from the post-collision lattice
+// P.src; ghost cells deeper than the first layer hold 0.
+
+__device__ __forceinline__ double faithful_get(const FusedParams &P, int q, int i, int j, int k)
+{
+    const int64_t r = ring_index(P, i, j, k);
+    if (r == -2) return ((const double *)P.src)[(uint64_t)q * P.pstride + ((uint64_t)k * P.ny + j) * P.nx + i];
+    if (r == -1) return 0.0;
+    return P.ring[(uint64_t)q * P.ring_n + (uint64_t)r];
+}
+
+// The one-rank unpack of lid_driven_cavity/src/mpi_view_transfer.cpp:617-688
+// writes receive buffers nothing filled (zeros) into the ghost planes
+// y = ny, z = nz, x = -1 and x = nx, all 27 populations: here their cells of
+// the first layer.  Thread t: the two x faces, then the plane y = ny (x in
+// [0, nx)), then the plane z = nz (x in [0, nx), y in [-1, ny]).
+__global__ void k_faithful_unpack(const FusedParams P)
+{
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nyz = (uint32_t)(P.ny + 2) * (P.nz + 2), nxz = (uint32_t)P.nx * (P.nz + 2),
+                   nxy = (uint32_t)P.nx * (P.ny + 2);
+    uint32_t t = t0;
+    int i, j, k;
+    if (t < 2 * nyz) {
+        const uint32_t f = t / nyz, r = t - f * nyz;
+        i = f ? P.nx : -1;
+        k = (int)(r / (P.ny + 2)) - 1;
+        j = (int)(r % (P.ny + 2)) - 1;
+    } else if ((t -= 2 * nyz) < nxz) {
+        j = P.ny;
+        k = (int)(t / P.nx) - 1;
+        i = (int)(t % P.nx);
+    } else if ((t -= nxz) < nxy) {
+        k = P.nz;
+        j = (int)(t / P.nx) - 1;
+        i = (int)(t % P.nx);
+    } else {
+        return;
+    }
+    const uint64_t r = (uint64_t)ring_index(P, i, j, k);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) P.ring[(uint64_t)q * P.ring_n + r] = 0.0;
+}
+
+// One wall of lid_driven_cavity/src/lbm.cpp:233-301: for the first-layer
+// cells (a, b) of the plane and every direction q pointing into the cavity,
+// f_q(plane) = f_bb(q)(src_plane, a + sa e_a, b + sb e_b).  The sources lie
+// in src_plane and the targets in plane, so all loads come first.
+template <int AXIS, int SIGN>
+__global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, int sa, int sb)
+{
+    const int la = AXIS == 0 ? P.ny : P.nx, lb = AXIS == 2 ? P.ny : P.nz;
+    const int na = la + 2, nb = lb + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int A = t % na - 1, B = t / na - 1;
+    double v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e[3] = {EX[q], EY[q], EZ[q]};
+        if (e[AXIS] * SIGN <= 0) continue;
+        int si, sj, sk;
+        if (AXIS == 0) {
+            si = src_plane; sj = A + sa * e[1]; sk = B + sb * e[2];
+        } else if (AXIS == 1) {
+            si = A + sa * e[0]; sj = src_plane; sk = B + sb * e[2];
+        } else {
+            si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
+        }
+        v[q] = faithful_get(P, BB[q], si, sj, sk);
+    }
+    int di, dj, dk;
+    if (AXIS == 0) {
+        di = plane; dj = A; dk = B;
+    } else if (AXIS == 1) {
+        di = A; dj = plane; dk = B;
+    } else {
+        di = A; dj = B; dk = plane;
+    }
+    const uint64_t r = (uint64_t)ring_index(P, di, dj, dk);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int e[3] = {EX[q], EY[q], EZ[q]};
+        if (e[AXIS] * SIGN <= 0) continue;
+        P.ring[(uint64_t)q * P.ring_n + r] = v[q];
+    }
+}
+
+hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st)
+{
+    const uint32_t nun = 2u * (P.ny + 2) * (P.nz + 2) + (uint32_t)P.nx * (P.nz + 2) + (uint32_t)P.nx * (P.ny + 2);
+    hipLaunchKernelGGL(k_faithful_unpack, dim3((nun + 255) / 256), dim3(256), 0, st, P);
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    auto grid = [](int la, int lb) { return dim3((unsigned)(((la + 2) * (lb + 2) + 255) / 256)); };
+    // bcl bcr bcf bcb bcd bcu; the top wall reads j + e_y (:298)
+    hipLaunchKernelGGL((k_faithful_wall<0, 1>), grid(ny, nz), dim3(256), 0, st, P, -1, 1, 2, 2);
+    hipLaunchKernelGGL((k_faithful_wall<0, -1>), grid(ny, nz), dim3(256), 0, st, P, nx, nx - 2, 2, 2);
+    hipLaunchKernelGGL((k_faithful_wall<1, 1>), grid(nx, nz), dim3(256), 0, st, P, -1, 1, 2, 2);
+    hipLaunchKernelGGL((k_faithful_wall<1, -1>), grid(nx, nz), dim3(256), 0, st, P, ny, ny - 2, 2, 2);
+    hipLaunchKernelGGL((k_faithful_wall<2, 1>), grid(nx, ny), dim3(256), 0, st, P, -1, 1, 2, 2);
+    hipLaunchKernelGGL((k_faithful_wall<2, -1>), grid(nx, ny), dim3(256), 0, st, P, nz, nz - 2, 2, 1);
+    return hipGetLastError();
+}
+
 }  // namespace imx

@@ -38,6 +38,8 @@ struct imexlb_handle {
     double corr[Q] = {};
     double feq_in[Q] = {};
     double *tgv_tab = nullptr;
+    double *ring = nullptr;  // faithful cavity: the first ghost layer (FusedParams::ring)
+    uint32_t ring_a = 0, ring_b = 0, ring_c = 0;
     int block = 256, cpt = 1, xcd = 1, ntl = 0;
     int64_t steps = 0;
     int64_t mom_step = -1;   // step whose moments are in mom[]
@@ -141,9 +143,12 @@ static int validate(const imexlb_config *c)
     if (c->cells_per_thread < 0 || c->cells_per_thread > 2)
         return fail(nullptr, IMEXLB_EINVAL, "cells_per_thread %d", c->cells_per_thread);
     if (c->engine == IMEXLB_ENGINE_FUSED) {
-        if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id != IMEXLB_CASE_TGV)
+        if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id == IMEXLB_CASE_SPHERE)
+            return fail(nullptr, IMEXLB_EUNSUPPORTED, "the faithful sphere runs on the phased engine");
+        if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id == IMEXLB_CASE_CAVITY &&
+            (c->dtype != IMEXLB_F64 || c->storage != IMEXLB_PULL))
             return fail(nullptr, IMEXLB_EUNSUPPORTED,
-                        "faithful cavity/sphere boundaries run on the phased engine");
+                        "the faithful cavity on the fused engine is FP64 with pull storage");
         if (c->case_id == IMEXLB_CASE_SPHERE && c->storage == IMEXLB_AA)
             return fail(nullptr, IMEXLB_EUNSUPPORTED, "the open channel runs with pull storage");
         if (c->case_id == IMEXLB_CASE_SPHERE) {
@@ -183,6 +188,11 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.ntl = h->ntl;
     P.divx.init(c.nx);
     P.divy.init(c.ny);
+    P.ring = h->ring;
+    P.ring_a = h->ring_a;
+    P.ring_b = h->ring_b;
+    P.ring_c = h->ring_c;
+    P.ring_n = 2 * (h->ring_a + h->ring_b + h->ring_c);
 }
 
 namespace imx {
@@ -199,7 +209,8 @@ hipError_t launch_fused(int dtype, int bc, int pattern, const FusedParams &P, in
 
 static int bc_of(const imexlb_config &c)
 {
-    return c.case_id == IMEXLB_CASE_TGV ? BC_PERIODIC : (c.case_id == IMEXLB_CASE_CAVITY ? BC_CAVITY : BC_CHANNEL);
+    if (c.case_id == IMEXLB_CASE_CAVITY) return c.mode == IMEXLB_MODE_FAITHFUL ? BC_FAITHFUL : BC_CAVITY;
+    return c.case_id == IMEXLB_CASE_TGV ? BC_PERIODIC : BC_CHANNEL;
 }
 
 // One fused step: step index k = h->steps + 1.
@@ -223,6 +234,9 @@ static int fused_step(imexlb_handle *h, bool write_moments)
         P.mom_v = h->mom[2];
         P.mom_w = h->mom[3];
     }
+    // the faithful cavity's unpack and walls fill the ghost ring from the
+    // post-collision lattice first, as the reference's Streaming does
+    if (bc_of(c) == BC_FAITHFUL) HIPCHK(h, launch_faithful_boundaries(P, h->stream));
     HIPCHK(h, launch_fused(c.dtype, bc_of(c), pattern, P, h->block, h->stream));
     if (c.storage == IMEXLB_PULL) h->cur = 1 - h->cur;
     h->steps++;
@@ -358,6 +372,14 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
     if (hipMalloc(&h->partials, sizeof(double) * 4 * h->nparts) != hipSuccess ||
         hipMalloc(&h->result, sizeof(double) * 4) != hipSuccess)
         return bail(fail(h, IMEXLB_ENOMEM, "reduction scratch"));
+    if (bc_of(c) == BC_FAITHFUL) {
+        h->ring_a = (uint32_t)(c.ny + 2) * (uint32_t)(c.nz + 2);
+        h->ring_b = (uint32_t)c.nx * (uint32_t)(c.nz + 2);
+        h->ring_c = (uint32_t)c.nx * (uint32_t)c.ny;
+        const size_t rb = (size_t)Q * 2 * (h->ring_a + h->ring_b + h->ring_c) * 8;
+        if (hipMalloc(&h->ring, rb) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "ghost ring"));
+        h->device_bytes += (int64_t)rb;
+    }
 
     // boundary constants in the oracle's operation order
     const double cs2 = 1.0 / 3.0;
@@ -444,6 +466,7 @@ int imexlb_destroy(imexlb_handle *h)
     if (h->partials) hipFree(h->partials);
     if (h->result) hipFree(h->result);
     if (h->tgv_tab) hipFree(h->tgv_tab);
+    if (h->ring) hipFree(h->ring);
     if (h->ph) phased_destroy(h->ph);
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
@@ -478,13 +501,16 @@ int imexlb_initialize(imexlb_handle *h)
     P.u0 = c.case_id == IMEXLB_CASE_SPHERE ? c.u0 : (c.case_id == IMEXLB_CASE_CAVITY ? 0.0 : c.u0);
     P.rho0 = c.rho0;
     P.cs2 = 1.0 / 3.0;
-    P.p0 = c.rho0 * P.cs2;
+    // the reference's cavity starts from p = 0, so f = 0 (lid_driven_cavity/src/lbm.cpp:196)
+    P.p0 = bc_of(c) == BC_FAITHFUL ? 0.0 : c.rho0 * P.cs2;
     P.shift = h->shift;
     P.mask = h->mask;
     for (int k = 0; k < 4; ++k) P.mom[k] = h->mom[k];
     P.divx.init(c.nx);
     P.divy.init(c.ny);
     HIPCHK(h, launch_init(c.dtype, P, h->stream));
+    if (h->ring)  // ghost cells start at f = 0 too
+        HIPCHK(h, hipMemsetAsync(h->ring, 0, (size_t)Q * 2 * (h->ring_a + h->ring_b + h->ring_c) * 8, h->stream));
     h->cur = 0;
     if (c.storage == IMEXLB_PULL && c.case_id == IMEXLB_CASE_SPHERE) {
         // solid cells are never written by the step; give both lattices the
@@ -712,6 +738,8 @@ int imexlb_checkpoint_write(imexlb_handle *h, const char *path)
 {
     if (!h || !path) return IMEXLB_EINVAL;
     if (!h->initialized) return fail(h, IMEXLB_ESTATE, "checkpoint before Initialize");
+    if (h->ring)
+        return fail(h, IMEXLB_EUNSUPPORTED, "restart files of the faithful cavity hold ghost cells: use the phased engine");
     int rc = set_device(h);
     if (rc) return rc;
     // fused: the post-collision populations of the interior; phased: the
@@ -733,6 +761,8 @@ int imexlb_checkpoint_write(imexlb_handle *h, const char *path)
 int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
 {
     if (!h || !path) return IMEXLB_EINVAL;
+    if (h->ring)
+        return fail(h, IMEXLB_EUNSUPPORTED, "restart files of the faithful cavity hold ghost cells: use the phased engine");
     if (!h->initialized) {
         int rc = imexlb_initialize(h);
         if (rc) return rc;

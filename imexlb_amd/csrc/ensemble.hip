@@ -48,7 +48,7 @@ __device__ __forceinline__ void tile_run(const FusedParams &P, uint32_t first, b
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         if (!k[r].live) continue;
-        cell_collide<S>(P, k[r], f[r], wm);
+        cell_collide<S, BC>(P, k[r], f[r], wm);
         cell_scatter<S, BC, PAT>(P, k[r], f[r]);
     }
 }

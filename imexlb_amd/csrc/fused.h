@@ -233,6 +233,7 @@ struct CellCtx {
     bool x0, x1, y0, y1, z0, z1;
     uint8_t m;            // channel mask of the cell
     bool live;
+    uint32_t rx0, ry0, rz0;  // BC_FAITHFUL: ring offsets of (y, z), (x, z), (x, y) on the x, y, z faces
 };
 
 template <int BC>
@@ -263,7 +264,7 @@ __device__ __forceinline__ void cell_setup(const FusedParams &P, uint32_t c, Cel
         k.xv[0] = x == 0 ? 0 : x - 1;
         k.xv[2] = x == nx - 1 ? nx - 1 : x + 1;
     }
-    if constexpr (BC == BC_CAVITY) {
+    if constexpr (BC == BC_CAVITY || BC == BC_FAITHFUL) {
         yv[0] = y == 0 ? 0 : y - 1;
         yv[2] = y == ny - 1 ? ny - 1 : y + 1;
         zv[0] = z == 0 ? 0 : z - 1;
@@ -287,6 +288,11 @@ __device__ __forceinline__ void cell_setup(const FusedParams &P, uint32_t c, Cel
     k.y1 = y == ny - 1;
     k.z0 = z == 0;
     k.z1 = z == nz - 1;
+    if constexpr (BC == BC_FAITHFUL) {
+        k.rx0 = (z + 1) * (ny + 2) + y + 1;
+        k.ry0 = (z + 1) * nx + x;
+        k.rz0 = y * nx + x;
+    }
 }
 
 // is the cell x - e_i outside the cavity?
@@ -351,6 +357,22 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
                     v = k.z1 ? lid : v;
                 }
                 f[i] = v;
+            } else if constexpr (BC == BC_FAITHFUL) {  // pull only
+                // a source outside the cavity is a ghost cell of the first
+                // layer, which the reference's wall kernels filled this step:
+                // read it from the ring (selects, as above)
+                const bool ox = (EX[i] > 0 && k.x0) || (EX[i] < 0 && k.x1);
+                const bool oy = (EY[i] > 0 && k.y0) || (EY[i] < 0 && k.y1);
+                const bool oz = (EZ[i] > 0 && k.z0) || (EZ[i] < 0 && k.z1);
+                const uint32_t nx = P.nx, ny = P.ny;
+                const uint32_t rxi = (EX[i] < 0 ? P.ring_a : 0u) + k.rx0 - EZ[i] * (ny + 2) - EY[i];
+                const uint32_t ryi = 2 * P.ring_a + (EY[i] < 0 ? P.ring_b : 0u) + k.ry0 - EZ[i] * nx - EX[i];
+                const uint32_t rzi =
+                    2 * (P.ring_a + P.ring_b) + (EZ[i] < 0 ? P.ring_c : 0u) + k.rz0 - EY[i] * nx - EX[i];
+                const uint32_t r = ox ? rxi : (oy ? ryi : rzi);
+                const S *p = (ox || oy || oz) ? (const S *)P.ring + (uint64_t)i * P.ring_n + r
+                                              : src + (uint64_t)i * ps + src_off(k, i);
+                f[i] = *glb(p);
             } else {  // channel, pull only
                 // selects as above; only the mask byte of a neighbour of a
                 // solid cell is loaded under a branch (its value picks the address)
@@ -371,7 +393,7 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
     }
 }
 
-template <typename S>
+template <typename S, int BC>
 __device__ __forceinline__ void cell_collide(const FusedParams &P, const CellCtx &k,
                                              typename RealOf<S>::type (&f)[Q], bool write_moments)
 {
@@ -380,6 +402,9 @@ __device__ __forceinline__ void cell_collide(const FusedParams &P, const CellCtx
     if constexpr (F64) {
         double pm, um, vm, wm;
         moments_f64(f, pm, um, vm, wm);
+        // the reference's lid: Update sets ua = 0.1 on the top fluid layer
+        // (lid_driven_cavity/src/lbm.cpp:343-344)
+        if constexpr (BC == BC_FAITHFUL) um = k.z1 ? 0.1 : um;
         if (write_moments) {
             *glb(P.mom_p + c) = pm;
             *glb(P.mom_u + c) = um;
@@ -450,7 +475,7 @@ __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         if (!k[r].live) continue;
-        cell_collide<S>(P, k[r], f[r], P.mom_p != nullptr);
+        cell_collide<S, BC>(P, k[r], f[r], P.mom_p != nullptr);
         cell_scatter<S, BC, PAT>(P, k[r], f[r]);
     }
 }
@@ -468,9 +493,17 @@ hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block
         if (pattern == PAT_PULL) IMX_L(BC_CAVITY, PAT_PULL);
         else if (pattern == PAT_AA_EVEN) IMX_L(BC_CAVITY, PAT_AA_EVEN);
         else IMX_L(BC_CAVITY, PAT_AA_ODD);
-    } else {
+    } else if (bc == BC_CHANNEL) {
         if (pattern != PAT_PULL) return hipErrorInvalidValue;
         IMX_L(BC_CHANNEL, PAT_PULL);
+    } else {
+        // the faithful cavity: FP64, pull storage
+        if constexpr (std::is_same<S, double>::value) {
+            if (pattern != PAT_PULL) return hipErrorInvalidValue;
+            IMX_L(BC_FAITHFUL, PAT_PULL);
+        } else {
+            return hipErrorInvalidValue;
+        }
     }
 #undef IMX_L
     return hipGetLastError();

@@ -16,7 +16,9 @@
 
 namespace imx {
 
-enum Bc : int { BC_PERIODIC = 0, BC_CAVITY = 1, BC_CHANNEL = 2 };
+// BC_FAITHFUL: the cavity with the reference's own wall, lid and halo rules;
+// the first ghost layer lives in a ring of face arrays (FusedParams::ring).
+enum Bc : int { BC_PERIODIC = 0, BC_CAVITY = 1, BC_CHANNEL = 2, BC_FAITHFUL = 3 };
 enum Pattern : int { PAT_PULL = 0, PAT_AA_EVEN = 1, PAT_AA_ODD = 2 };
 enum Dtype : int { DT_F64 = 0, DT_F32 = 1, DT_F16 = 2 };
 
@@ -63,7 +65,33 @@ struct FusedParams {
     int ntl;                 // 1: non-temporal loads of unshared lines
     int bc;                  // boundary kind (read by the ensemble kernel)
     FastDiv divx, divy;
+    // BC_FAITHFUL: the first ghost layer, population-major (q * ring_n + r).
+    // Cell r of the ring: the two x faces (x = -1, nx; y in [-1, ny], z in
+    // [-1, nz]; ring_a cells each) first, then the two y faces (y = -1, ny;
+    // x in [0, nx), z in [-1, nz]; ring_b each), then the two z faces (z =
+    // -1, nz; x in [0, nx), y in [0, ny); ring_c each), each x fastest.
+    double *ring;
+    uint32_t ring_n, ring_a, ring_b, ring_c;
 };
+
+// The ring index of ghost cell (i, j, k), -2 for an interior cell and -1 for
+// a ghost cell deeper than the first layer (those stay 0 in the faithful
+// cavity: nothing writes them and Initialize makes f = 0).
+__host__ __device__ __forceinline__ int64_t ring_index(const FusedParams &P, int i, int j, int k)
+{
+    if (i < -1 || i > P.nx || j < -1 || j > P.ny || k < -1 || k > P.nz) return -1;
+    if (i == -1 || i == P.nx)
+        return (i == P.nx ? (int64_t)P.ring_a : 0) + (int64_t)(k + 1) * (P.ny + 2) + (j + 1);
+    if (j == -1 || j == P.ny)
+        return 2 * (int64_t)P.ring_a + (j == P.ny ? (int64_t)P.ring_b : 0) + (int64_t)(k + 1) * P.nx + i;
+    if (k == -1 || k == P.nz)
+        return 2 * (int64_t)(P.ring_a + P.ring_b) + (k == P.nz ? (int64_t)P.ring_c : 0) + (int64_t)j * P.nx + i;
+    return -2;
+}
+
+// The faithful cavity's unpack and wall kernels on the ring (aux.hip):
+// Streaming's boundary phase, before the fused pass.
+hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st);
 
 // Launchers (defined next to their kernels).
 hipError_t launch_fused(int dtype, int bc, int pattern, const FusedParams &P, int block, hipStream_t st);

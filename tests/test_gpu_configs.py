@@ -75,6 +75,23 @@ def test_cavity128_faithful_phased_first_steps_bitexact():
     assert np.array_equal(o.get_f(), g.get_f())
 
 
+def test_cavity128_faithful_fused_first_steps_bitexact():
+    """The reference's own cavity rules on the fused engine at 128^3: equal to
+    the oracle after 40 steps, populations included."""
+    from imexlb_amd import LBM
+    n = 128
+    o = Oracle(LDC, n, n, n, tau0=0.096, u0=0.1, mode=FAITHFUL, lid_u=0.1)
+    o.Initialize()
+    g = LBM(n, n, n, 0.096, u0=0.1, case="cavity", mode="faithful", engine="fused")
+    g.Initialize()
+    print("[cavity128 faithful fused] oracle 40 steps", flush=True)
+    o.run(40)
+    g.step(40)
+    fields_equal(o, g, "cavity128 faithful fused")
+    o.Collision()
+    assert np.array_equal(o.get_f(), g.get_f())
+
+
 def test_sphere_channel_first_steps_bitexact():
     """512 x 256 x 256 channel, sphere D = 40 at (128, 128, 128), inlet 0.05,
     Re 200 (tau0 = 3 * 0.05 * 40 / 200): fused pull equals the oracle on the
