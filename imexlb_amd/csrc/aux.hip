@@ -66,6 +66,20 @@ __global__ void k_init(const InitParams P)
         v = -P.u0 * cx * sy * cz;
         w = 0;
         p = P.rho0 * P.cs2 + P.rho0 * P.u0 * P.u0 / 16.0 * (c2x + c2y) * (c2z + 2.0);
+    } else if (P.kind == 2) {
+        // sphere/src/lbm.cpp:224-239: p = 0, a parabolic x velocity in z
+        // masked by usr and perturbed by urand(100) (here the seeded hash of
+        // the cell's index in the reference's ghost-3 layout)
+        const uint64_t ly = (uint64_t)P.ny + 6, lz = (uint64_t)P.nz + 6;
+        const uint64_t idx = ((uint64_t)(x + 3) * ly + (y + 3)) * lz + (z + 3);
+        const int ran = (int)(mix64(P.seed + idx * 0x9E3779B97F4A7C15ULL) % 100ULL);
+        const int usr = faithful_sphere_solid(P.nx, (int)x, (int)y, (int)z) ? 0 : 1;
+        const int glz = P.nz, kz = (int)z;
+        const double gm = (double)(glz - 1);
+        u = (P.u0 * 4.0 * (double)kz * (double)(glz - 1 - kz) / (gm * gm)) * (double)usr * (1.0 + (double)ran * 0.001);
+        v = 0.0;
+        w = 0.0;
+        p = 0.0;
     } else {
         const bool solid = P.mask && P.mask[c] == 1;
         u = solid ? 0.0 : P.u0;
@@ -322,17 +336,87 @@ hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// The faithful cavity on the fused engine: Streaming's boundary phase on the
-// ring (the first ghost layer), in the reference's order, before the fused
-// pass reads it.  Interior values come from the post-collision lattice
-// P.src; ghost cells deeper than the first layer hold 0.
+// Faithful mode on the fused engine: Streaming's boundary phase on the ring
+// (the first ghost layer), in the reference's order, before the fused pass
+// reads it.  Interior values come from the post-collision lattice P.src.
+// Ghost cells deeper than the first layer are never written, so they keep
+// Initialize's values: 0 in the cavity, whose fields start at rest with
+// p = 0; in the sphere channel the initial fields reach into the ghosts
+// (sphere/src/lbm.cpp:224-239 loops over the whole ghosted box), and those
+// values are recomputed where a wall reads them.
 
-__device__ __forceinline__ double faithful_get(const FusedParams &P, int q, int i, int j, int k)
+// the faithful sphere's initial population q at cell (i, j, k) of the
+// ghosted box, in interior coordinates (k_init's kind 2 at any cell)
+__device__ __forceinline__ double faithful_sphere_f0(const FusedParams &P, uint64_t seed, int q, int i, int j, int k)
+{
+    const uint64_t ly = (uint64_t)P.ny + 6, lz = (uint64_t)P.nz + 6;
+    const uint64_t idx = ((uint64_t)(i + 3) * ly + (uint64_t)(j + 3)) * lz + (uint64_t)(k + 3);
+    const int ran = (int)(mix64(seed + idx * 0x9E3779B97F4A7C15ULL) % 100ULL);
+    const int usr = faithful_sphere_solid(P.nx, i, j, k) ? 0 : 1;
+    const int glz = P.nz, kz = k;
+    const double gm = (double)(glz - 1);
+    const double u = (P.u0 * 4.0 * (double)kz * (double)(glz - 1 - kz) / (gm * gm)) * (double)usr * (1.0 + (double)ran * 0.001);
+    return feq_f64(q, 0.0, u, 0.0, 0.0);
+}
+
+// Where population q of cell (i, j, k) is read from, in two steps so that a
+// kernel can first place all its sources (the deep-ghost value is computed
+// under a branch, with no load in it), then issue all its loads back to back
+// (a value loaded before a divergent branch and still live after it would be
+// waited for there), then pick.
+struct GhostSrc {
+    const double *p;  // always a valid address
+    double deep;      // the value when the cell lies deeper than the first layer
+    bool is_deep;
+};
+
+__device__ __forceinline__ GhostSrc faithful_src(const FusedParams &P, int q, int i, int j, int k)
 {
     const int64_t r = ring_index(P, i, j, k);
-    if (r == -2) return ((const double *)P.src)[(uint64_t)q * P.pstride + ((uint64_t)k * P.ny + j) * P.nx + i];
-    if (r == -1) return 0.0;
-    return P.ring[(uint64_t)q * P.ring_n + (uint64_t)r];
+    GhostSrc g;
+    g.is_deep = r == -1;
+    g.deep = 0.0;
+    if (g.is_deep && P.faithful_sphere) g.deep = faithful_sphere_f0(P, P.seed, q, i, j, k);
+    const uint64_t in_cell = r == -2 ? ((uint64_t)k * P.ny + j) * P.nx + i : 0;
+    g.p = r == -2 ? (const double *)P.src + (uint64_t)q * P.pstride + in_cell
+                  : P.ring + (uint64_t)q * P.ring_n + (uint64_t)(r < 0 ? 0 : r);
+    return g;
+}
+
+// Initialize for the ring: every first-layer ghost cell takes its initial
+// populations (0 in the cavity).
+__global__ void k_faithful_ring_init(const FusedParams P)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P.ring_n) return;
+    const uint32_t rr = r;
+    int i, j, k;
+    if (r < 2 * P.ring_a) {
+        i = r < P.ring_a ? -1 : P.nx;
+        r -= r < P.ring_a ? 0 : P.ring_a;
+        j = (int)(r % (P.ny + 2)) - 1;
+        k = (int)(r / (P.ny + 2)) - 1;
+    } else if ((r -= 2 * P.ring_a) < 2 * P.ring_b) {
+        j = r < P.ring_b ? -1 : P.ny;
+        r -= r < P.ring_b ? 0 : P.ring_b;
+        i = (int)(r % P.nx);
+        k = (int)(r / P.nx) - 1;
+    } else {
+        r -= 2 * P.ring_b;
+        k = r < P.ring_c ? -1 : P.nz;
+        r -= r < P.ring_c ? 0 : P.ring_c;
+        i = (int)(r % P.nx);
+        j = (int)(r / P.nx);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        P.ring[(uint64_t)q * P.ring_n + rr] = P.faithful_sphere ? faithful_sphere_f0(P, P.seed, q, i, j, k) : 0.0;
+}
+
+hipError_t launch_faithful_ring_init(const FusedParams &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_faithful_ring_init, dim3((P.ring_n + 255) / 256), dim3(256), 0, st, P);
+    return hipGetLastError();
 }
 
 // The one-rank unpack of lid_driven_cavity/src/mpi_view_transfer.cpp:617-688
@@ -380,6 +464,7 @@ __global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, i
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na * nb) return;
     const int A = t % na - 1, B = t / na - 1;
+    GhostSrc g[Q];
     double v[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -393,8 +478,14 @@ __global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, i
         } else {
             si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
         }
-        v[q] = faithful_get(P, BB[q], si, sj, sk);
+        g[q] = faithful_src(P, BB[q], si, sj, sk);
     }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if ((AXIS == 0 ? EX[q] : AXIS == 1 ? EY[q] : EZ[q]) * SIGN > 0) v[q] = *g[q].p;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if ((AXIS == 0 ? EX[q] : AXIS == 1 ? EY[q] : EZ[q]) * SIGN > 0) v[q] = g[q].is_deep ? g[q].deep : v[q];
     int di, dj, dk;
     if (AXIS == 0) {
         di = plane; dj = A; dk = B;
@@ -412,12 +503,100 @@ __global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, i
     }
 }
 
+// sphere/src/lbm.cpp:326-347: the inlet ghost plane takes the first
+// interior plane's populations with e_x > 0, the outlet ghost plane takes
+// f_q(nx - 1, y + e_y, z + e_z) for e_x < 0; first-layer cells only.
+template <int OUTLET>
+__global__ void k_faithful_inlet_outlet(const FusedParams P)
+{
+    const int na = P.ny + 2, nb = P.nz + 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na * nb) return;
+    const int j = t % na - 1, k = t / na - 1;
+    GhostSrc g[Q];
+    double v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (!OUTLET && EX[q] > 0) g[q] = faithful_src(P, q, 0, j, k);
+        if (OUTLET && EX[q] < 0) g[q] = faithful_src(P, q, P.nx - 1, j + EY[q], k + EZ[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if ((!OUTLET && EX[q] > 0) || (OUTLET && EX[q] < 0)) v[q] = *g[q].p;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if ((!OUTLET && EX[q] > 0) || (OUTLET && EX[q] < 0)) v[q] = g[q].is_deep ? g[q].deep : v[q];
+    const uint64_t r = (uint64_t)ring_index(P, OUTLET ? P.nx : -1, j, k);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if ((!OUTLET && EX[q] > 0) || (OUTLET && EX[q] < 0)) P.ring[(uint64_t)q * P.ring_n + r] = v[q];
+}
+
+// "usrbc", sphere/src/lbm.cpp:350-356: a solid cell takes, for each
+// direction q whose neighbour x + e_q is fluid, f_bb(q)(x + 2 e_q), in place
+// in the post-collision lattice.  Ghost cells are fluid (the ball never
+// reaches them).  A source is never a target: between two solid voxels of a
+// ball the midpoint voxel is solid too, so that direction is not taken.
+// Thread t walks the box [lo, lo + n)^3 around the ball (clamped to the
+// cavity).  Every thread loads (threads off the box or on fluid cells from
+// a valid cell) and only solid cells store: no load sits under a divergent
+// branch, where it would be waited for at once.
+__global__ void k_faithful_usr(const FusedParams P, int lo, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tt = t < n * n * n ? t : 0;
+    const int x0 = lo + tt % n, y0 = lo + (tt / n) % n, z0 = lo + tt / (n * n);
+    const bool valid = (t < n * n * n) & (x0 < P.nx) & (y0 < P.ny) & (z0 < P.nz);
+    const int x = valid ? x0 : 0, y = valid ? y0 : 0, z = valid ? z0 : 0;
+    const uint32_t c = ((uint32_t)z * P.ny + y) * P.nx + x;
+    const uint8_t mc = P.mask[c];  // (bitwise logic below: no short-circuit branch around a load)
+    bool in[Q];
+    uint64_t mi[Q];
+    uint8_t m[Q];
+    GhostSrc g[Q];
+    double v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int xi = x + EX[q], yi = y + EY[q], zi = z + EZ[q];
+        in[q] = (xi >= 0) & (xi < P.nx) & (yi >= 0) & (yi < P.ny) & (zi >= 0) & (zi < P.nz);
+        mi[q] = in[q] ? ((uint64_t)zi * P.ny + yi) * P.nx + xi : (uint64_t)c;
+        g[q] = faithful_src(P, BB[q], x + 2 * EX[q], y + 2 * EY[q], z + 2 * EZ[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        m[q] = P.mask[mi[q]];
+        v[q] = *g[q].p;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) v[q] = g[q].is_deep ? g[q].deep : v[q];
+    const bool solid = valid & (mc == 1);
+    double *lat = (double *)P.src;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (solid & (!in[q] | (m[q] != 1))) lat[(uint64_t)q * P.pstride + c] = v[q];
+}
+
 hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st)
 {
-    const uint32_t nun = 2u * (P.ny + 2) * (P.nz + 2) + (uint32_t)P.nx * (P.nz + 2) + (uint32_t)P.nx * (P.ny + 2);
-    hipLaunchKernelGGL(k_faithful_unpack, dim3((nun + 255) / 256), dim3(256), 0, st, P);
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     auto grid = [](int la, int lb) { return dim3((unsigned)(((la + 2) * (lb + 2) + 255) / 256)); };
+    if (P.faithful_sphere) {
+        // bcf bcb bcd bcu, inlet, outlet, usrbc (sphere/src/lbm.cpp:277-356);
+        // one rank: the unpack has nothing to write
+        hipLaunchKernelGGL((k_faithful_wall<1, 1>), grid(nx, nz), dim3(256), 0, st, P, -1, 1, 2, 2);
+        hipLaunchKernelGGL((k_faithful_wall<1, -1>), grid(nx, nz), dim3(256), 0, st, P, ny, ny - 2, 2, 2);
+        hipLaunchKernelGGL((k_faithful_wall<2, 1>), grid(nx, ny), dim3(256), 0, st, P, -1, 1, 2, 2);
+        hipLaunchKernelGGL((k_faithful_wall<2, -1>), grid(nx, ny), dim3(256), 0, st, P, nz, nz - 2, 2, 2);
+        hipLaunchKernelGGL(k_faithful_inlet_outlet<0>, grid(ny, nz), dim3(256), 0, st, P);
+        hipLaunchKernelGGL(k_faithful_inlet_outlet<1>, grid(ny, nz), dim3(256), 0, st, P);
+        // the ball lies within [c - r, c + r] of its centre c on every axis
+        const int cc = (nx - 1) / 8, rr = (int)std::ceil((nx - 1) / 64.0);
+        const int lo = cc - rr < 0 ? 0 : cc - rr, n = cc + rr - lo + 1;
+        hipLaunchKernelGGL(k_faithful_usr, dim3((unsigned)((n * n * n + 255) / 256)), dim3(256), 0, st, P, lo, n);
+        return hipGetLastError();
+    }
+    const uint32_t nun = 2u * (P.ny + 2) * (P.nz + 2) + (uint32_t)P.nx * (P.nz + 2) + (uint32_t)P.nx * (P.ny + 2);
+    hipLaunchKernelGGL(k_faithful_unpack, dim3((nun + 255) / 256), dim3(256), 0, st, P);
     // bcl bcr bcf bcb bcd bcu; the top wall reads j + e_y (:298)
     hipLaunchKernelGGL((k_faithful_wall<0, 1>), grid(ny, nz), dim3(256), 0, st, P, -1, 1, 2, 2);
     hipLaunchKernelGGL((k_faithful_wall<0, -1>), grid(ny, nz), dim3(256), 0, st, P, nx, nx - 2, 2, 2);

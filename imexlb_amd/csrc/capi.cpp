@@ -143,15 +143,13 @@ static int validate(const imexlb_config *c)
     if (c->cells_per_thread < 0 || c->cells_per_thread > 2)
         return fail(nullptr, IMEXLB_EINVAL, "cells_per_thread %d", c->cells_per_thread);
     if (c->engine == IMEXLB_ENGINE_FUSED) {
-        if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id == IMEXLB_CASE_SPHERE)
-            return fail(nullptr, IMEXLB_EUNSUPPORTED, "the faithful sphere runs on the phased engine");
-        if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id == IMEXLB_CASE_CAVITY &&
+        if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id != IMEXLB_CASE_TGV &&
             (c->dtype != IMEXLB_F64 || c->storage != IMEXLB_PULL))
             return fail(nullptr, IMEXLB_EUNSUPPORTED,
-                        "the faithful cavity on the fused engine is FP64 with pull storage");
+                        "the faithful cavity and sphere on the fused engine are FP64 with pull storage");
         if (c->case_id == IMEXLB_CASE_SPHERE && c->storage == IMEXLB_AA)
             return fail(nullptr, IMEXLB_EUNSUPPORTED, "the open channel runs with pull storage");
-        if (c->case_id == IMEXLB_CASE_SPHERE) {
+        if (c->case_id == IMEXLB_CASE_SPHERE && c->mode == IMEXLB_MODE_CONFIG) {
             const double r = c->sphere_r;
             if (!(r > 0) || c->sphere_cx - r < 2 || c->sphere_cx + r > c->nx - 3 || c->sphere_cy - r < 1 ||
                 c->sphere_cy + r > c->ny - 2 || c->sphere_cz - r < 1 || c->sphere_cz + r > c->nz - 2)
@@ -193,6 +191,9 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.ring_b = h->ring_b;
     P.ring_c = h->ring_c;
     P.ring_n = 2 * (h->ring_a + h->ring_b + h->ring_c);
+    P.faithful_sphere = c.case_id == IMEXLB_CASE_SPHERE && c.mode == IMEXLB_MODE_FAITHFUL;
+    P.u0 = c.u0;
+    P.seed = c.seed;
 }
 
 namespace imx {
@@ -209,8 +210,9 @@ hipError_t launch_fused(int dtype, int bc, int pattern, const FusedParams &P, in
 
 static int bc_of(const imexlb_config &c)
 {
-    if (c.case_id == IMEXLB_CASE_CAVITY) return c.mode == IMEXLB_MODE_FAITHFUL ? BC_FAITHFUL : BC_CAVITY;
-    return c.case_id == IMEXLB_CASE_TGV ? BC_PERIODIC : BC_CHANNEL;
+    if (c.case_id == IMEXLB_CASE_TGV) return BC_PERIODIC;
+    if (c.mode == IMEXLB_MODE_FAITHFUL) return BC_FAITHFUL;
+    return c.case_id == IMEXLB_CASE_CAVITY ? BC_CAVITY : BC_CHANNEL;
 }
 
 // One fused step: step index k = h->steps + 1.
@@ -386,7 +388,18 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
     if (c.case_id == IMEXLB_CASE_CAVITY) {
         for (int i = 0; i < Q; ++i) h->corr[i] = 6.0 * w(i) * c.rho0 * ((double)EX[i] * c.lid_u);
     }
-    if (c.case_id == IMEXLB_CASE_SPHERE) {
+    if (c.case_id == IMEXLB_CASE_SPHERE && c.mode == IMEXLB_MODE_FAITHFUL) {
+        // the reference's ball (sphere/src/lbm.cpp:238): 1 solid, 0 fluid
+        std::vector<uint8_t> m((size_t)h->ncell, 0);
+        for (int z = 0; z < c.nz; ++z)
+            for (int y = 0; y < c.ny; ++y)
+                for (int x = 0; x < c.nx; ++x)
+                    if (faithful_sphere_solid(c.nx, x, y, z)) m[((size_t)z * c.ny + y) * c.nx + x] = 1;
+        if (hipMalloc(&h->mask, h->ncell) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "mask"));
+        h->device_bytes += h->ncell;
+        if (hipMemcpy(h->mask, m.data(), h->ncell, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(h, IMEXLB_EHIP, "mask upload"));
+    } else if (c.case_id == IMEXLB_CASE_SPHERE) {
         const double pin = c.rho0 * cs2;
         for (int i = 0; i < Q; ++i) {
             const double u = c.u0, v = 0.0, ww = 0.0;
@@ -496,7 +509,8 @@ int imexlb_initialize(imexlb_handle *h)
     P.pstride = h->pstride;
     P.dst = h->lat[0];
     P.layout = c.storage == IMEXLB_AA ? 1 : 0;
-    P.kind = c.case_id == IMEXLB_CASE_TGV ? 0 : 1;
+    P.kind = c.case_id == IMEXLB_CASE_TGV ? 0 : (bc_of(c) == BC_FAITHFUL && c.case_id == IMEXLB_CASE_SPHERE ? 2 : 1);
+    P.seed = c.seed;
     P.tab = h->tgv_tab;
     P.u0 = c.case_id == IMEXLB_CASE_SPHERE ? c.u0 : (c.case_id == IMEXLB_CASE_CAVITY ? 0.0 : c.u0);
     P.rho0 = c.rho0;
@@ -509,8 +523,11 @@ int imexlb_initialize(imexlb_handle *h)
     P.divx.init(c.nx);
     P.divy.init(c.ny);
     HIPCHK(h, launch_init(c.dtype, P, h->stream));
-    if (h->ring)  // ghost cells start at f = 0 too
-        HIPCHK(h, hipMemsetAsync(h->ring, 0, (size_t)Q * 2 * (h->ring_a + h->ring_b + h->ring_c) * 8, h->stream));
+    if (h->ring) {  // the first ghost layer starts from the initial fields too
+        FusedParams F;
+        fill_fused_params(h, F, 0, 0);
+        HIPCHK(h, launch_faithful_ring_init(F, h->stream));
+    }
     h->cur = 0;
     if (c.storage == IMEXLB_PULL && c.case_id == IMEXLB_CASE_SPHERE) {
         // solid cells are never written by the step; give both lattices the
@@ -821,7 +838,7 @@ int imexlb_diagnostics(imexlb_handle *h, double *ke, double *mass)
 int imexlb_sphere_force(imexlb_handle *h, double *force3)
 {
     if (!h || !force3) return IMEXLB_EINVAL;
-    if (h->ph || h->cfg.case_id != IMEXLB_CASE_SPHERE)
+    if (h->ph || h->cfg.case_id != IMEXLB_CASE_SPHERE || h->cfg.mode != IMEXLB_MODE_CONFIG)
         return fail(h, IMEXLB_EUNSUPPORTED, "sphere force is defined for the config-mode channel");
     int rc = set_device(h);
     if (rc) return rc;

@@ -234,6 +234,7 @@ struct CellCtx {
     uint8_t m;            // channel mask of the cell
     bool live;
     uint32_t rx0, ry0, rz0;  // BC_FAITHFUL: ring offsets of (y, z), (x, z), (x, y) on the x, y, z faces
+    uint32_t z;              // BC_FAITHFUL: the cell's z
 };
 
 template <int BC>
@@ -289,6 +290,7 @@ __device__ __forceinline__ void cell_setup(const FusedParams &P, uint32_t c, Cel
     k.z0 = z == 0;
     k.z1 = z == nz - 1;
     if constexpr (BC == BC_FAITHFUL) {
+        k.z = z;
         k.rx0 = (z + 1) * (ny + 2) + y + 1;
         k.ry0 = (z + 1) * nx + x;
         k.rz0 = y * nx + x;
@@ -402,9 +404,28 @@ __device__ __forceinline__ void cell_collide(const FusedParams &P, const CellCtx
     if constexpr (F64) {
         double pm, um, vm, wm;
         moments_f64(f, pm, um, vm, wm);
-        // the reference's lid: Update sets ua = 0.1 on the top fluid layer
-        // (lid_driven_cavity/src/lbm.cpp:343-344)
-        if constexpr (BC == BC_FAITHFUL) um = k.z1 ? 0.1 : um;
+        if constexpr (BC == BC_FAITHFUL) {
+            if (P.faithful_sphere) {
+                // sphere/src/lbm.cpp:399-412: u, p *= usr; the inlet plane
+                // takes the parabolic profile in z, the outlet plane p = 0
+                const double m = P.mask[c] == 1 ? 0.0 : 1.0;
+                um = um * m;
+                vm = vm * m;
+                wm = wm * m;
+                pm = pm * m;
+                const int glz = P.nz, kz = (int)k.z;
+                const double gm = (double)(glz - 1);
+                const double uin = P.u0 * 4.0 * (double)kz * (double)(glz - 1 - kz) / (gm * gm);
+                um = k.x0 ? uin : um;
+                vm = k.x0 ? 0.0 : vm;
+                wm = k.x0 ? 0.0 : wm;
+                pm = k.x1 ? 0.0 : pm;
+            } else {
+                // the reference's lid: Update sets ua = 0.1 on the top fluid
+                // layer (lid_driven_cavity/src/lbm.cpp:343-344)
+                um = k.z1 ? 0.1 : um;
+            }
+        }
         if (write_moments) {
             *glb(P.mom_p + c) = pm;
             *glb(P.mom_u + c) = um;

@@ -22,6 +22,19 @@ enum Bc : int { BC_PERIODIC = 0, BC_CAVITY = 1, BC_CHANNEL = 2, BC_FAITHFUL = 3 
 enum Pattern : int { PAT_PULL = 0, PAT_AA_EVEN = 1, PAT_AA_ODD = 2 };
 enum Dtype : int { DT_F64 = 0, DT_F32 = 1, DT_F16 = 2 };
 
+// The faithful sphere's inlet perturbation replaces Kokkos' random pool by a
+// seeded counter hash: the splitmix64 finaliser (Steele, Lea and Flood, "Fast
+// splittable pseudorandom number generators", OOPSLA 2014).
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBULL;
+    x ^= x >> 31;
+    return x;
+}
+
 // Division by a run-time invariant 32-bit divisor (Granlund & Montgomery,
 // "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
 // exact for every 32-bit dividend.
@@ -72,26 +85,42 @@ struct FusedParams {
     // -1, nz; x in [0, nx), y in [0, ny); ring_c each), each x fastest.
     double *ring;
     uint32_t ring_n, ring_a, ring_b, ring_c;
+    int faithful_sphere;     // BC_FAITHFUL: 0 the cavity, 1 the sphere channel
+    double u0;               // faithful sphere: inlet speed of the parabolic profile
+    uint64_t seed;           // faithful sphere: the initial perturbation's seed
 };
 
 // The ring index of ghost cell (i, j, k), -2 for an interior cell and -1 for
 // a ghost cell deeper than the first layer (those stay 0 in the faithful
 // cavity: nothing writes them and Initialize makes f = 0).
+// (selects rather than branches: a kernel's loads stay free of waits)
 __host__ __device__ __forceinline__ int64_t ring_index(const FusedParams &P, int i, int j, int k)
 {
-    if (i < -1 || i > P.nx || j < -1 || j > P.ny || k < -1 || k > P.nz) return -1;
-    if (i == -1 || i == P.nx)
-        return (i == P.nx ? (int64_t)P.ring_a : 0) + (int64_t)(k + 1) * (P.ny + 2) + (j + 1);
-    if (j == -1 || j == P.ny)
-        return 2 * (int64_t)P.ring_a + (j == P.ny ? (int64_t)P.ring_b : 0) + (int64_t)(k + 1) * P.nx + i;
-    if (k == -1 || k == P.nz)
-        return 2 * (int64_t)(P.ring_a + P.ring_b) + (k == P.nz ? (int64_t)P.ring_c : 0) + (int64_t)j * P.nx + i;
-    return -2;
+    const bool deep = i < -1 || i > P.nx || j < -1 || j > P.ny || k < -1 || k > P.nz;
+    const bool xf = i == -1 || i == P.nx, yf = j == -1 || j == P.ny, zf = k == -1 || k == P.nz;
+    const int64_t rx = (i == P.nx ? (int64_t)P.ring_a : 0) + (int64_t)(k + 1) * (P.ny + 2) + (j + 1);
+    const int64_t ry = 2 * (int64_t)P.ring_a + (j == P.ny ? (int64_t)P.ring_b : 0) + (int64_t)(k + 1) * P.nx + i;
+    const int64_t rz =
+        2 * (int64_t)(P.ring_a + P.ring_b) + (k == P.nz ? (int64_t)P.ring_c : 0) + (int64_t)j * P.nx + i;
+    return deep ? -1 : (xf ? rx : (yf ? ry : (zf ? rz : -2)));
 }
 
 // The faithful cavity's unpack and wall kernels on the ring (aux.hip):
 // Streaming's boundary phase, before the fused pass.
 hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st);
+// Initialize's values for the ring (0 in the cavity)
+hipError_t launch_faithful_ring_init(const FusedParams &P, hipStream_t st);
+
+// The faithful sphere: solid iff (x-c)^2 + (y-c)^2 + (z-c)^2 <= r^2 with the
+// integer centre c = (glx - 1) / 8 on every axis and r = (glx - 1) / 64
+// (sphere/src/lbm.cpp:238).
+__host__ __device__ __forceinline__ bool faithful_sphere_solid(int glx, int x, int y, int z)
+{
+    const int c = (glx - 1) / 8;
+    const int di = x - c, dj = y - c, dk = z - c;
+    const double r = (glx - 1) / 64.0;
+    return (double)di * (double)di + (double)dj * (double)dj + (double)dk * (double)dk <= r * r;
+}
 
 // Launchers (defined next to their kernels).
 hipError_t launch_fused(int dtype, int bc, int pattern, const FusedParams &P, int block, hipStream_t st);
@@ -102,11 +131,13 @@ struct InitParams {
     uint64_t pstride;
     void *dst;               // lattice written (stored convention)
     int layout;              // 0 natural (pull), 1 swapped (AA after an even step)
-    int kind;                // 0 TGV from the per-axis tables, 1 uniform (u0,0,0)/p0, masked
+    int kind;                // 0 TGV from the per-axis tables, 1 uniform (u0,0,0)/p0, masked,
+                             // 2 the faithful sphere (parabolic inlet profile, seeded perturbation)
     const double *tab;       // TGV: 9 tables [sx cx c2x | sy cy c2y | sz cz c2z]
     double u0, rho0, cs2, p0;
     double shift;
     const uint8_t *mask;
+    uint64_t seed;           // kind 2
     double *mom[4];          // p, u, v, w of the initial fields (solid cells: 0);
                              // null: not written
     FastDiv divx, divy;

@@ -76,16 +76,6 @@ __device__ __forceinline__ double feq_ref(int q, double p, double u, double v, d
     return WVAL[WCLASS[q]] * p * 3.0 + WVAL[WCLASS[q]] * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
 }
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x)
-{
-    x ^= x >> 30;
-    x *= 0xBF58476D1CE4E5B9ULL;
-    x ^= x >> 27;
-    x *= 0x94D049BB133111EBULL;
-    x ^= x >> 31;
-    return x;
-}
-
 struct Arrays {
     double *f, *ft, *ua, *va, *wa, *p;
     int *usr;

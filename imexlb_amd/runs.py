@@ -27,7 +27,7 @@ timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, a fix
 phased: the reference's own boundary code (faithful mode) on the phased
     engine's multi-step path: the cavity at n^3 (lid 0.1, tau0 0.096) and the
     sphere input of sphere/input.in (512 x 128 x 128, Re 100, u0 0.05); and the
-    same cavity on the fused engine (ghost ring + one fused pass); GLUPS over
+    same two on the fused engine (ghost ring + one fused pass); GLUPS over
     `steps` timed steps after 3 warm-up steps.
 """
 import argparse
@@ -261,8 +261,11 @@ def phased(a):
     cases = {f"cavity {n}^3": dict(shape=(n, n, n), tau=0.096, u0=0.1, case="cavity", engine="phased"),
              "sphere 512x128x128 Re 100": dict(shape=(512, 128, 128), tau=0.05 * 2 * r / 100.0 / (1.0 / 3.0),
                                                u0=0.05, case="sphere", engine="phased"),
-             f"cavity {n}^3 fused engine": dict(shape=(n, n, n), tau=0.096, u0=0.1, case="cavity", engine="fused")}
-    out = {"workload": "faithful mode, phased engine (and the cavity on the fused engine)", "runs": {}}
+             f"cavity {n}^3 fused engine": dict(shape=(n, n, n), tau=0.096, u0=0.1, case="cavity", engine="fused"),
+             "sphere 512x128x128 Re 100 fused engine": dict(shape=(512, 128, 128),
+                                                            tau=0.05 * 2 * r / 100.0 / (1.0 / 3.0), u0=0.05,
+                                                            case="sphere", engine="fused")}
+    out = {"workload": "faithful mode, phased and fused engines", "runs": {}}
     for name, c in cases.items():
         g = LBM(*c["shape"], c["tau"], u0=c["u0"], case=c["case"], mode="faithful", engine=c["engine"])
         if c["engine"] == "fused":

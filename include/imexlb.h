@@ -48,9 +48,10 @@ extern "C" {
 
 /* engines */
 #define IMEXLB_ENGINE_FUSED 0  /* one kernel per step: stream+moments+collide
-                                  (TGV, config modes, and the faithful cavity in
-                                  FP64 pull storage, whose first ghost layer is
-                                  filled by the reference's wall kernels first) */
+                                  (TGV, config modes, and the faithful cavity and
+                                  sphere in FP64 pull storage, whose first ghost
+                                  layer the reference's boundary kernels fill
+                                  first) */
 #define IMEXLB_ENGINE_PHASED 1 /* the reference's phases as separate kernels on
                                   its ghost-3 layout (FP64, all cases, faithful) */
 

@@ -6,7 +6,7 @@ oracle, FP64 bit for bit."""
 import numpy as np
 import pytest
 
-from oracle.oracle import CONFIG, FAITHFUL, LDC, TGV, Oracle
+from oracle.oracle import CONFIG, FAITHFUL, LDC, SPHERE, TGV, Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,30 @@ def test_cavity128_faithful_fused_first_steps_bitexact():
     fields_equal(o, g, "cavity128 faithful fused")
     o.Collision()
     assert np.array_equal(o.get_f(), g.get_f())
+
+
+def test_reference_sphere_input_faithful_both_engines():
+    """The reference's own sphere input (sphere/input.in: 512 x 128 x 128,
+    Re 100, u0 0.05, tau0 = 3 u0 2R / Re with R = 511 / 64) with its own
+    rules, on the phased and on the fused engine: both equal the oracle after
+    4 steps."""
+    from oracle.oracle import SPHERE
+    from imexlb_amd import LBM
+    nx, ny, nz = 512, 128, 128
+    r = (nx - 1) / 64.0
+    tau = 0.05 * 2 * r / 100.0 / (1.0 / 3.0)
+    o = Oracle(SPHERE, nx, ny, nz, tau0=tau, u0=0.05, mode=FAITHFUL)
+    o.Initialize()
+    print("[sphere input] oracle 4 steps", flush=True)
+    o.run(4)
+    mo = o.moments()
+    for engine in ("phased", "fused"):
+        g = LBM(nx, ny, nz, tau, u0=0.05, case="sphere", mode="faithful", engine=engine)
+        g.Initialize()
+        g.step(4)
+        for name, a, b in zip("puvw", mo, g.moments()):
+            assert np.array_equal(a, b), f"sphere input {engine}: moment {name} differs"
+        g.close()
 
 
 def test_sphere_channel_first_steps_bitexact():

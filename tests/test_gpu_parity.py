@@ -424,6 +424,22 @@ def test_fused_faithful_cavity_bitexact(n):
     assert same(o.get_f(), g.get_f())
 
 
+@pytest.mark.parametrize("n", [(40, 14, 12), (24, 10, 9), (72, 20, 20), (9, 3, 4)])
+def test_fused_faithful_sphere_bitexact(n):
+    """The reference's own sphere rules on the fused engine: walls on y and z,
+    the inlet and outlet ghost copies and usrbc fill the ghost ring (and the
+    solid cells), one fused pass streams and applies usr, the parabolic inlet
+    and p = 0 at the outlet.  Equal to the oracle bit for bit."""
+    o, g = make_pair(SPHERE, FAITHFUL, n, 0.08, 0.1, engine="fused")
+    moments_equal(o, g)
+    for k in (1, 8, 31):
+        o.run(k)
+        g.step(k)
+        moments_equal(o, g)
+    o.Collision()
+    assert same(o.get_f(), g.get_f())
+
+
 def test_fused_faithful_cavity_six_calls_and_launch_shapes(tmp_path):
     """The six calls per step, and every launch shape, give the same bits;
     restart files are refused (they would need the ghost ring)."""

@@ -95,7 +95,7 @@ def test_create_rejects_bad_configs_before_touching_a_device():
     bad = [dict(sx=1), dict(tau=-0.6), dict(dtype="f32", engine="phased"),
            dict(case="cavity", mode="faithful", engine="fused", storage="aa"),
            dict(case="cavity", mode="faithful", engine="fused", storage="pull", dtype="f32"),
-           dict(case="sphere", mode="faithful", engine="fused"),
+           dict(case="sphere", mode="faithful", engine="fused", storage="aa"),
            dict(case="sphere", mode="config", storage="aa", sphere=(20, 8, 8, 3)),
            dict(case="sphere", mode="config", sphere=(2, 8, 8, 3)), dict(block=100)]
     for kw in bad:
