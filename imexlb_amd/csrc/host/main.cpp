@@ -46,6 +46,7 @@ int main(int argc, char **argv)
     bool honour_input = false, fast = false, have_sphere = false, output = false;
     std::string tune, restart, checkpoint;
     int place = 1;
+    bool engine_given = false;
     for (int a = 1; a < argc; ++a) {
         std::string k = argv[a];
         auto val = [&]() -> std::string {
@@ -57,7 +58,10 @@ int main(int argc, char **argv)
         };
         if (k == "--case") opt.case_id = pick(val(), {{"tgv", 0}, {"cavity", 1}, {"ldc", 1}, {"sphere", 2}}, "case");
         else if (k == "--mode") opt.mode = pick(val(), {{"faithful", 0}, {"config", 1}}, "mode");
-        else if (k == "--engine") opt.engine = pick(val(), {{"fused", 0}, {"phased", 1}}, "engine");
+        else if (k == "--engine") {
+            opt.engine = pick(val(), {{"fused", 0}, {"phased", 1}}, "engine");
+            engine_given = true;
+        }
         else if (k == "--dtype") opt.dtype = pick(val(), {{"f64", 0}, {"f32", 1}, {"f16", 2}}, "dtype");
         else if (k == "--storage") opt.storage = pick(val(), {{"pull", 0}, {"aa", 1}}, "storage");
         else if (k == "--input") input = val();
@@ -87,7 +91,10 @@ int main(int argc, char **argv)
             return 2;
         }
     }
-    if (opt.case_id != IMEXLB_CASE_TGV && opt.mode == IMEXLB_MODE_FAITHFUL) opt.engine = IMEXLB_ENGINE_PHASED;
+    // the reference's own cavity and sphere run phase by phase unless the
+    // fused engine (same bits, no restart files) is asked for
+    if (opt.case_id != IMEXLB_CASE_TGV && opt.mode == IMEXLB_MODE_FAITHFUL && !engine_given)
+        opt.engine = IMEXLB_ENGINE_PHASED;
 
     try {
         System s1(input, opt.case_id, !honour_input);

@@ -79,6 +79,19 @@ def test_driver_restart_faithful_cavity(tmp_path):
     assert (ke, mass) == (ke_full, mass_full)
 
 
+def test_driver_faithful_cavity_fused_engine(tmp_path):
+    """--engine fused runs the reference's own cavity rules on the fused
+    engine's ghost ring, to the final KE and mass of the phased engine.  The
+    fields are the same bits (tests/test_gpu_parity.py); the two engines'
+    FP64 reductions add them in different orders, so the sums agree to
+    rounding."""
+    text = "1.0  0.1  10\n0.1  15  0\n20 16 12\n"
+    common = ["--case", "cavity", "--mode", "faithful", "--honour-input"]
+    ke_p, mass_p, _ = run(tmp_path, text, common)
+    ke_f, mass_f, _ = run(tmp_path, text, common + ["--engine", "fused"])
+    assert abs(ke_f - ke_p) <= 1e-14 * ke_p and abs(mass_f - mass_p) <= 1e-14 * abs(mass_p)
+
+
 def test_driver_output_files(tmp_path):
     """--output writes output0.plt (the initial fields) and one file every
     `inter` steps, as the cavity driver does (lid_driven_cavity/main.cpp:26,
