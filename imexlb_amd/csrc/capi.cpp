@@ -750,25 +750,37 @@ int imexlb_set_f(imexlb_handle *h, const double *f)
 
 static const char CK_MAGIC[8] = {'I', 'M', 'X', 'L', 'B', 'C', 'K', '1'};
 static const char CK_MAGIC_PHASED[8] = {'I', 'M', 'X', 'L', 'B', 'P', 'H', '1'};
+// the fused engine's faithful runs: the populations, then the ghost ring
+static const char CK_MAGIC_RING[8] = {'I', 'M', 'X', 'L', 'B', 'C', 'R', '1'};
+
+static size_t ring_doubles(const imexlb_handle *h)
+{
+    return h->ring ? (size_t)Q * 2 * (h->ring_a + h->ring_b + h->ring_c) : 0;
+}
 
 int imexlb_checkpoint_write(imexlb_handle *h, const char *path)
 {
     if (!h || !path) return IMEXLB_EINVAL;
     if (!h->initialized) return fail(h, IMEXLB_ESTATE, "checkpoint before Initialize");
-    if (h->ring)
-        return fail(h, IMEXLB_EUNSUPPORTED, "restart files of the faithful cavity hold ghost cells: use the phased engine");
     int rc = set_device(h);
     if (rc) return rc;
-    // fused: the post-collision populations of the interior; phased: the
-    // engine's whole state, ghost cells and moments included
-    const size_t n = h->ph ? (size_t)phased_state_doubles(h->ph) : (size_t)Q * h->ncell;
+    // fused: the post-collision populations of the interior (and, in faithful
+    // mode, the ghost ring after them); phased: the engine's whole state,
+    // ghost cells and moments included
+    const size_t nf = h->ph ? (size_t)phased_state_doubles(h->ph) : (size_t)Q * h->ncell;
+    const size_t n = nf + ring_doubles(h);
     std::vector<double> f(n);
     rc = h->ph ? phased_get_state(h->ph, f.data(), h->err) : imexlb_get_f(h, f.data());
     if (rc) return rc;
+    if (h->ring) {
+        HIPCHK(h, hipMemcpyAsync(f.data() + nf, h->ring, ring_doubles(h) * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
     FILE *fp = std::fopen(path, "wb");
     if (!fp) return fail(h, IMEXLB_EIO, "cannot open %s", path);
     const int64_t steps = h->steps;
-    bool ok = std::fwrite(h->ph ? CK_MAGIC_PHASED : CK_MAGIC, 1, 8, fp) == 8 &&
+    const char *magic = h->ph ? CK_MAGIC_PHASED : (h->ring ? CK_MAGIC_RING : CK_MAGIC);
+    bool ok = std::fwrite(magic, 1, 8, fp) == 8 &&
               std::fwrite(&h->cfg, sizeof h->cfg, 1, fp) == 1 && std::fwrite(&steps, 8, 1, fp) == 1 &&
               std::fwrite(f.data(), 8, n, fp) == n;
     if (std::fclose(fp) != 0) ok = false;
@@ -778,8 +790,6 @@ int imexlb_checkpoint_write(imexlb_handle *h, const char *path)
 int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
 {
     if (!h || !path) return IMEXLB_EINVAL;
-    if (h->ring)
-        return fail(h, IMEXLB_EUNSUPPORTED, "restart files of the faithful cavity hold ghost cells: use the phased engine");
     if (!h->initialized) {
         int rc = imexlb_initialize(h);
         if (rc) return rc;
@@ -789,21 +799,29 @@ int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
     char magic[8];
     imexlb_config c;
     int64_t steps = 0;
-    const size_t n = h->ph ? (size_t)phased_state_doubles(h->ph) : (size_t)Q * h->ncell;
+    const size_t nf = h->ph ? (size_t)phased_state_doubles(h->ph) : (size_t)Q * h->ncell;
+    const size_t n = nf + ring_doubles(h);
     std::vector<double> f(n);
     bool ok = std::fread(magic, 1, 8, fp) == 8 && std::fread(&c, sizeof c, 1, fp) == 1 &&
               std::fread(&steps, 8, 1, fp) == 1;
-    const bool phased_file = ok && std::memcmp(magic, CK_MAGIC_PHASED, 8) == 0;
-    if (ok && !phased_file && std::memcmp(magic, CK_MAGIC, 8) != 0) ok = false;
-    if (ok && phased_file != (h->ph != nullptr)) {
+    const char *want = h->ph ? CK_MAGIC_PHASED : (h->ring ? CK_MAGIC_RING : CK_MAGIC);
+    if (ok && std::memcmp(magic, want, 8) != 0) {
         std::fclose(fp);
-        return fail(h, IMEXLB_EINVAL, "checkpoint %s holds the %s engine's state", path,
-                    phased_file ? "phased" : "fused");
+        const bool known = std::memcmp(magic, CK_MAGIC, 8) == 0 || std::memcmp(magic, CK_MAGIC_PHASED, 8) == 0 ||
+                           std::memcmp(magic, CK_MAGIC_RING, 8) == 0;
+        if (!known) return fail(h, IMEXLB_EIO, "%s is not a restart file", path);
+        return fail(h, IMEXLB_EINVAL, "checkpoint %s holds another engine's or mode's state", path);
     }
     if (ok && (c.case_id != h->cfg.case_id || c.mode != h->cfg.mode || c.nx != h->cfg.nx || c.ny != h->cfg.ny ||
                c.nz != h->cfg.nz)) {
         std::fclose(fp);
         return fail(h, IMEXLB_EINVAL, "checkpoint %s is for another case or extent", path);
+    }
+    // the fused faithful sphere recomputes its deeper ghost cells from the
+    // initial fields on every step, which depend on u0 and the seed
+    if (ok && h->ring && h->cfg.case_id == IMEXLB_CASE_SPHERE && (c.u0 != h->cfg.u0 || c.seed != h->cfg.seed)) {
+        std::fclose(fp);
+        return fail(h, IMEXLB_EINVAL, "checkpoint %s is for another u0 or seed", path);
     }
     ok = ok && steps >= 0 && std::fread(f.data(), 8, n, fp) == n;
     std::fclose(fp);
@@ -814,6 +832,10 @@ int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
     if (h->ph) {
         h->phase_collided = false;
         return phased_set_state(h->ph, f.data(), h->err);
+    }
+    if (h->ring) {
+        HIPCHK(h, hipMemcpyAsync(h->ring, f.data() + nf, ring_doubles(h) * 8, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     return imexlb_set_f(h, f.data());
 }

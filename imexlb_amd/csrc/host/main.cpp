@@ -92,7 +92,7 @@ int main(int argc, char **argv)
         }
     }
     // the reference's own cavity and sphere run phase by phase unless the
-    // fused engine (same bits, no restart files) is asked for
+    // fused engine (same bits) is asked for
     if (opt.case_id != IMEXLB_CASE_TGV && opt.mode == IMEXLB_MODE_FAITHFUL && !engine_given)
         opt.engine = IMEXLB_ENGINE_PHASED;
 

@@ -158,7 +158,11 @@ int imexlb_set_f(imexlb_handle *h, const double *f);
  * configuration, the step count, then the engine's whole state between steps
  * as FP64 -- f, p, ua, va, wa over the ghosted box, since the faithful wall
  * rules read ghost values written in earlier steps; a phased handle of the
- * same case, mode and extents reads it.  Either resumes at the saved step. */
+ * same case, mode and extents reads it.  The fused engine's faithful
+ * cavity and sphere: "IMXLBCR1", as "IMXLBCK1" followed by the ghost ring
+ * (27 populations over the six first-layer ghost faces, FP64); a sphere
+ * handle must also have the same u0 and seed.  Each kind of
+ * handle refuses the other kinds' files.  All resume at the saved step. */
 int imexlb_checkpoint_write(imexlb_handle *h, const char *path);
 int imexlb_checkpoint_read(imexlb_handle *h, const char *path);
 

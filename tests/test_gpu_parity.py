@@ -467,8 +467,37 @@ def test_fused_faithful_cavity_six_calls_and_launch_shapes(tmp_path):
             ref = m
         for a, b in zip(ref, m):
             assert same(a, b), shape
+
+
+@pytest.mark.parametrize("case,n", [(LDC, (12, 10, 14)), (SPHERE, (40, 14, 12))])
+def test_fused_faithful_checkpoint_restart_continues_bit_for_bit(tmp_path, case, n):
+    """A fused faithful restart file holds the post-collision populations and
+    the ghost ring, whose wall values persist from step to step: a handle
+    restarted from it runs on to the oracle's fields, bit for bit.  Handles of
+    the other engine, or of the fused engine's periodic mode, refuse it; so
+    does a sphere handle of another seed, whose deeper ghost cells differ."""
+    from imexlb_amd import LBM, ImexlbError
+    o, a = make_pair(case, FAITHFUL, n, 0.08, 0.1, engine="fused")
+    o.run(6)
+    a.step(6)
+    path = tmp_path / "fu.ck"
+    a.checkpoint(path)
+    a.close()
+    b = LBM(*n, 0.08, u0=0.1, case=case, mode=FAITHFUL, engine="fused")
+    b.restart(path)
+    assert b.info()["steps"] == 6
+    o.run(7)
+    b.step(7)
+    moments_equal(o, b)
+    o.Collision()
+    assert same(o.get_f(), b.get_f())
     with pytest.raises(ImexlbError):
-        g.checkpoint(tmp_path / "x.ck")
+        LBM(*n, 0.08, u0=0.1, case=case, mode=FAITHFUL, engine="phased").restart(path)
+    with pytest.raises(ImexlbError):
+        LBM(*n, 0.08, u0=0.1).restart(path)
+    if case == SPHERE:
+        with pytest.raises(ImexlbError):
+            LBM(*n, 0.08, u0=0.1, case=case, mode=FAITHFUL, engine="fused", seed=7).restart(path)
 
 
 def test_macro_is_three_p_and_u():
