@@ -187,6 +187,7 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.divx.init(c.nx);
     P.divy.init(c.ny);
     P.ring = h->ring;
+    if (h->ring) P.ring_rel = ((intptr_t)h->ring - (intptr_t)h->lat[src]) / (intptr_t)sizeof(double);
     P.ring_a = h->ring_a;
     P.ring_b = h->ring_b;
     P.ring_c = h->ring_c;

@@ -372,9 +372,13 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
                 const uint32_t rzi =
                     2 * (P.ring_a + P.ring_b) + (EZ[i] < 0 ? P.ring_c : 0u) + k.rz0 - EY[i] * nx - EX[i];
                 const uint32_t r = ox ? rxi : (oy ? ryi : rzi);
-                const S *p = (ox || oy || oz) ? (const S *)P.ring + (uint64_t)i * P.ring_n + r
-                                              : src + (uint64_t)i * ps + src_off(k, i);
-                f[i] = *glb(p);
+                // one base, two offsets: the ring's offset from the lattice
+                // may be negative, so the address is formed modulo 2^64
+                const uint64_t at = (ox || oy || oz) ? (uint64_t)P.ring_rel + (uint64_t)i * P.ring_n + r
+                                                     : (uint64_t)i * ps + src_off(k, i);
+                const S *p = (const S *)((uintptr_t)src + at * sizeof(S));
+                if (NTL && EX[i] == 0) f[i] = __builtin_nontemporal_load(glb(p));
+                else f[i] = *glb(p);
             } else {  // channel, pull only
                 // selects as above; only the mask byte of a neighbour of a
                 // solid cell is loaded under a branch (its value picks the address)

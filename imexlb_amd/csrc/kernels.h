@@ -84,6 +84,7 @@ struct FusedParams {
     // x in [0, nx), z in [-1, nz]; ring_b each), then the two z faces (z =
     // -1, nz; x in [0, nx), y in [0, ny); ring_c each), each x fastest.
     double *ring;
+    int64_t ring_rel;        // (ring - src) in elements: ring loads address from the lattice's base
     uint32_t ring_n, ring_a, ring_b, ring_c;
     int faithful_sphere;     // BC_FAITHFUL: 0 the cavity, 1 the sphere channel
     double u0;               // faithful sphere: inlet speed of the parabolic profile

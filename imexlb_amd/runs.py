@@ -265,11 +265,19 @@ def phased(a):
              "sphere 512x128x128 Re 100 fused engine": dict(shape=(512, 128, 128),
                                                             tau=0.05 * 2 * r / 100.0 / (1.0 / 3.0), u0=0.05,
                                                             case="sphere", engine="fused")}
-    out = {"workload": "faithful mode, phased and fused engines", "runs": {}}
+    out = {"workload": "faithful mode, phased and fused engines", "runs": {}, "placement": {}}
+    trials = 8 if a.placements is None else a.placements
     for name, c in cases.items():
         g = LBM(*c["shape"], c["tau"], u0=c["u0"], case=c["case"], mode="faithful", engine=c["engine"])
-        if c["engine"] == "fused":
-            g.autotune(None, steps=8)  # launch shape (the phased engine has none)
+        g.Initialize()
+        if c["engine"] == "fused":  # launch shape and placement (the phased engine keeps its own layout)
+            g.autotune(None, steps=8)
+            if trials > 1:
+                kept, ms_each = g.tune_placement(trials, steps=10)
+                cells = c["shape"][0] * c["shape"][1] * c["shape"][2]
+                out["placement"][name] = {"kept": kept, "glups_each": [
+                    round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
+                log(f"[phased] {name} placement: {out['placement'][name]}")
         g.Initialize()
         g.step(3)
         ms = g.step_timed(a.steps)

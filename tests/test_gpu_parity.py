@@ -470,6 +470,21 @@ def test_fused_faithful_cavity_six_calls_and_launch_shapes(tmp_path):
 
 
 @pytest.mark.parametrize("case,n", [(LDC, (12, 10, 14)), (SPHERE, (40, 14, 12))])
+def test_fused_faithful_placement_search_keeps_results(case, n):
+    """The placement search re-allocates the lattices of a faithful run (the
+    ghost ring stays) and re-initialises it: the run then equals the
+    oracle's, bit for bit."""
+    o, g = make_pair(case, FAITHFUL, n, 0.08, 0.1, engine="fused")
+    kept, ms = g.tune_placement(3, steps=2)
+    assert 0 <= kept < 3 and ms[0] > 0
+    o.run(9)
+    g.step(9)
+    moments_equal(o, g)
+    o.Collision()
+    assert same(o.get_f(), g.get_f())
+
+
+@pytest.mark.parametrize("case,n", [(LDC, (12, 10, 14)), (SPHERE, (40, 14, 12))])
 def test_fused_faithful_checkpoint_restart_continues_bit_for_bit(tmp_path, case, n):
     """A fused faithful restart file holds the post-collision populations and
     the ghost ring, whose wall values persist from step to step: a handle
