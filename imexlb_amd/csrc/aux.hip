@@ -452,10 +452,43 @@ __global__ void k_faithful_unpack(const FusedParams P)
     for (int q = 0; q < Q; ++q) P.ring[(uint64_t)q * P.ring_n + r] = 0.0;
 }
 
+// The d-th direction, in index order, of the nine with e_AXIS * SIGN > 0,
+// and its velocity and opposite.  The boundary kernels below give each such
+// direction its own row of workgroups (blockIdx.y): their (cell, direction)
+// pairs are independent, and nine times the lanes hide the latency of their
+// scattered loads.
+template <int AXIS, int SIGN>
+__device__ __forceinline__ int nth_dir(int d)
+{
+    int q = 0, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int e = AXIS == 0 ? EX[k] : AXIS == 1 ? EY[k] : EZ[k];
+        if (e * SIGN > 0) {
+            if (cnt == d) q = k;
+            ++cnt;
+        }
+    }
+    return q;
+}
+__device__ __forceinline__ void dir_of(int q, int (&e)[3], int &bq)
+{
+    e[0] = e[1] = e[2] = bq = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        if (k == q) {
+            e[0] = EX[k];
+            e[1] = EY[k];
+            e[2] = EZ[k];
+            bq = BB[k];
+        }
+}
+
 // One wall of lid_driven_cavity/src/lbm.cpp:233-301: for the first-layer
 // cells (a, b) of the plane and every direction q pointing into the cavity,
 // f_q(plane) = f_bb(q)(src_plane, a + sa e_a, b + sb e_b).  The sources lie
-// in src_plane and the targets in plane, so all loads come first.
+// in src_plane and the targets in plane, so no source is a target.
+// Thread t of row blockIdx.y: cell t of the plane, its blockIdx.y-th such q.
 template <int AXIS, int SIGN>
 __global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, int sa, int sb)
 {
@@ -463,123 +496,95 @@ __global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, i
     const int na = la + 2, nb = lb + 2;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na * nb) return;
+    const int q = nth_dir<AXIS, SIGN>(blockIdx.y);
+    int e[3], bq;
+    dir_of(q, e, bq);
     const int A = t % na - 1, B = t / na - 1;
-    GhostSrc g[Q];
-    double v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int e[3] = {EX[q], EY[q], EZ[q]};
-        if (e[AXIS] * SIGN <= 0) continue;
-        int si, sj, sk;
-        if (AXIS == 0) {
-            si = src_plane; sj = A + sa * e[1]; sk = B + sb * e[2];
-        } else if (AXIS == 1) {
-            si = A + sa * e[0]; sj = src_plane; sk = B + sb * e[2];
-        } else {
-            si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
-        }
-        g[q] = faithful_src(P, BB[q], si, sj, sk);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if ((AXIS == 0 ? EX[q] : AXIS == 1 ? EY[q] : EZ[q]) * SIGN > 0) v[q] = *g[q].p;
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if ((AXIS == 0 ? EX[q] : AXIS == 1 ? EY[q] : EZ[q]) * SIGN > 0) v[q] = g[q].is_deep ? g[q].deep : v[q];
-    int di, dj, dk;
+    int si, sj, sk, di, dj, dk;
     if (AXIS == 0) {
+        si = src_plane; sj = A + sa * e[1]; sk = B + sb * e[2];
         di = plane; dj = A; dk = B;
     } else if (AXIS == 1) {
+        si = A + sa * e[0]; sj = src_plane; sk = B + sb * e[2];
         di = A; dj = plane; dk = B;
     } else {
+        si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
         di = A; dj = B; dk = plane;
     }
+    const GhostSrc g = faithful_src(P, bq, si, sj, sk);
+    double v = *g.p;
+    v = g.is_deep ? g.deep : v;
     const uint64_t r = (uint64_t)ring_index(P, di, dj, dk);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int e[3] = {EX[q], EY[q], EZ[q]};
-        if (e[AXIS] * SIGN <= 0) continue;
-        P.ring[(uint64_t)q * P.ring_n + r] = v[q];
-    }
+    P.ring[(uint64_t)q * P.ring_n + r] = v;
 }
 
 // sphere/src/lbm.cpp:326-347: the inlet ghost plane takes the first
 // interior plane's populations with e_x > 0, the outlet ghost plane takes
-// f_q(nx - 1, y + e_y, z + e_z) for e_x < 0; first-layer cells only.
+// f_q(nx - 1, y + e_y, z + e_z) for e_x < 0; first-layer cells only (the
+// sources lie on x = 0 or nx - 1, the targets on x = -1 or nx).  Thread t of
+// row blockIdx.y: cell t of the plane, its blockIdx.y-th such q.
 template <int OUTLET>
 __global__ void k_faithful_inlet_outlet(const FusedParams P)
 {
     const int na = P.ny + 2, nb = P.nz + 2;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na * nb) return;
+    const int q = OUTLET ? nth_dir<0, -1>(blockIdx.y) : nth_dir<0, 1>(blockIdx.y);
+    int e[3], bq;
+    dir_of(q, e, bq);
     const int j = t % na - 1, k = t / na - 1;
-    GhostSrc g[Q];
-    double v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (!OUTLET && EX[q] > 0) g[q] = faithful_src(P, q, 0, j, k);
-        if (OUTLET && EX[q] < 0) g[q] = faithful_src(P, q, P.nx - 1, j + EY[q], k + EZ[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if ((!OUTLET && EX[q] > 0) || (OUTLET && EX[q] < 0)) v[q] = *g[q].p;
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if ((!OUTLET && EX[q] > 0) || (OUTLET && EX[q] < 0)) v[q] = g[q].is_deep ? g[q].deep : v[q];
+    const GhostSrc g = OUTLET ? faithful_src(P, q, P.nx - 1, j + e[1], k + e[2]) : faithful_src(P, q, 0, j, k);
+    double v = *g.p;
+    v = g.is_deep ? g.deep : v;
     const uint64_t r = (uint64_t)ring_index(P, OUTLET ? P.nx : -1, j, k);
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if ((!OUTLET && EX[q] > 0) || (OUTLET && EX[q] < 0)) P.ring[(uint64_t)q * P.ring_n + r] = v[q];
+    P.ring[(uint64_t)q * P.ring_n + r] = v;
 }
 
 // "usrbc", sphere/src/lbm.cpp:350-356: a solid cell takes, for each
 // direction q whose neighbour x + e_q is fluid, f_bb(q)(x + 2 e_q), in place
 // in the post-collision lattice.  Ghost cells are fluid (the ball never
 // reaches them).  A source is never a target: between two solid voxels of a
-// ball the midpoint voxel is solid too, so that direction is not taken.
-// Thread t walks the box [lo, lo + n)^3 around the ball (clamped to the
-// cavity).  Every thread loads (threads off the box or on fluid cells from
-// a valid cell) and only solid cells store: no load sits under a divergent
-// branch, where it would be waited for at once.
+// ball the midpoint voxel is solid too, so that direction is not taken.  The
+// (cell, q) pairs are therefore independent: thread t of row q = blockIdx.y
+// takes cell t of the box [lo, lo + n)^3 around the ball (clamped to the
+// cavity), so that the few thousand cells of the box fill 27 times as many
+// lanes.  Every thread loads (threads off the box from a valid cell) and only
+// solid cells store: no load sits under a divergent branch.
 __global__ void k_faithful_usr(const FusedParams P, int lo, int n)
 {
+    const int q = blockIdx.y;
+    int ex = 0, ey = 0, ez = 0, bq = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        if (k == q) {
+            ex = EX[k];
+            ey = EY[k];
+            ez = EZ[k];
+            bq = BB[k];
+        }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int tt = t < n * n * n ? t : 0;
     const int x0 = lo + tt % n, y0 = lo + (tt / n) % n, z0 = lo + tt / (n * n);
     const bool valid = (t < n * n * n) & (x0 < P.nx) & (y0 < P.ny) & (z0 < P.nz);
     const int x = valid ? x0 : 0, y = valid ? y0 : 0, z = valid ? z0 : 0;
     const uint32_t c = ((uint32_t)z * P.ny + y) * P.nx + x;
-    const uint8_t mc = P.mask[c];  // (bitwise logic below: no short-circuit branch around a load)
-    bool in[Q];
-    uint64_t mi[Q];
-    uint8_t m[Q];
-    GhostSrc g[Q];
-    double v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int xi = x + EX[q], yi = y + EY[q], zi = z + EZ[q];
-        in[q] = (xi >= 0) & (xi < P.nx) & (yi >= 0) & (yi < P.ny) & (zi >= 0) & (zi < P.nz);
-        mi[q] = in[q] ? ((uint64_t)zi * P.ny + yi) * P.nx + xi : (uint64_t)c;
-        g[q] = faithful_src(P, BB[q], x + 2 * EX[q], y + 2 * EY[q], z + 2 * EZ[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        m[q] = P.mask[mi[q]];
-        v[q] = *g[q].p;
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) v[q] = g[q].is_deep ? g[q].deep : v[q];
+    const int xi = x + ex, yi = y + ey, zi = z + ez;
+    const bool in = (xi >= 0) & (xi < P.nx) & (yi >= 0) & (yi < P.ny) & (zi >= 0) & (zi < P.nz);
+    const uint64_t mi = in ? ((uint64_t)zi * P.ny + yi) * P.nx + xi : (uint64_t)c;
+    const GhostSrc g = faithful_src(P, bq, x + 2 * ex, y + 2 * ey, z + 2 * ez);
+    const uint8_t mc = P.mask[c];
+    const uint8_t m = P.mask[mi];
+    double v = *g.p;
+    v = g.is_deep ? g.deep : v;
     const bool solid = valid & (mc == 1);
-    double *lat = (double *)P.src;
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if (solid & (!in[q] | (m[q] != 1))) lat[(uint64_t)q * P.pstride + c] = v[q];
+    if (solid & (!in | (m != 1))) ((double *)P.src)[(uint64_t)q * P.pstride + c] = v;
 }
 
 hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st)
 {
     const int nx = P.nx, ny = P.ny, nz = P.nz;
-    auto grid = [](int la, int lb) { return dim3((unsigned)(((la + 2) * (lb + 2) + 255) / 256)); };
+    // nine rows of workgroups: one per direction entering the domain
+    auto grid = [](int la, int lb) { return dim3((unsigned)(((la + 2) * (lb + 2) + 255) / 256), 9); };
     if (P.faithful_sphere) {
         // bcf bcb bcd bcu, inlet, outlet, usrbc (sphere/src/lbm.cpp:277-356);
         // one rank: the unpack has nothing to write
@@ -592,7 +597,7 @@ hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st)
         // the ball lies within [c - r, c + r] of its centre c on every axis
         const int cc = (nx - 1) / 8, rr = (int)std::ceil((nx - 1) / 64.0);
         const int lo = cc - rr < 0 ? 0 : cc - rr, n = cc + rr - lo + 1;
-        hipLaunchKernelGGL(k_faithful_usr, dim3((unsigned)((n * n * n + 255) / 256)), dim3(256), 0, st, P, lo, n);
+        hipLaunchKernelGGL(k_faithful_usr, dim3((unsigned)((n * n * n + 255) / 256), Q), dim3(256), 0, st, P, lo, n);
         return hipGetLastError();
     }
     const uint32_t nun = 2u * (P.ny + 2) * (P.nz + 2) + (uint32_t)P.nx * (P.nz + 2) + (uint32_t)P.nx * (P.ny + 2);
