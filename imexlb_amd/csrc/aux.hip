@@ -452,43 +452,13 @@ __global__ void k_faithful_unpack(const FusedParams P)
     for (int q = 0; q < Q; ++q) P.ring[(uint64_t)q * P.ring_n + r] = 0.0;
 }
 
-// The d-th direction, in index order, of the nine with e_AXIS * SIGN > 0,
-// and its velocity and opposite.  The boundary kernels below give each such
-// direction its own row of workgroups (blockIdx.y): their (cell, direction)
-// pairs are independent, and nine times the lanes hide the latency of their
-// scattered loads.
-template <int AXIS, int SIGN>
-__device__ __forceinline__ int nth_dir(int d)
-{
-    int q = 0, cnt = 0;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        const int e = AXIS == 0 ? EX[k] : AXIS == 1 ? EY[k] : EZ[k];
-        if (e * SIGN > 0) {
-            if (cnt == d) q = k;
-            ++cnt;
-        }
-    }
-    return q;
-}
-__device__ __forceinline__ void dir_of(int q, int (&e)[3], int &bq)
-{
-    e[0] = e[1] = e[2] = bq = 0;
-#pragma unroll
-    for (int k = 0; k < Q; ++k)
-        if (k == q) {
-            e[0] = EX[k];
-            e[1] = EY[k];
-            e[2] = EZ[k];
-            bq = BB[k];
-        }
-}
-
 // One wall of lid_driven_cavity/src/lbm.cpp:233-301: for the first-layer
 // cells (a, b) of the plane and every direction q pointing into the cavity,
 // f_q(plane) = f_bb(q)(src_plane, a + sa e_a, b + sb e_b).  The sources lie
 // in src_plane and the targets in plane, so no source is a target.
-// Thread t of row blockIdx.y: cell t of the plane, its blockIdx.y-th such q.
+// Thread t of row blockIdx.y: cell t of the plane, its blockIdx.y-th such q
+// (a row per direction: the pairs are independent, and nine times the lanes
+// hide the latency of the scattered loads).
 template <int AXIS, int SIGN>
 __global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, int sa, int sb)
 {
@@ -542,8 +512,10 @@ __global__ void k_faithful_inlet_outlet(const FusedParams P)
 
 // "usrbc", sphere/src/lbm.cpp:350-356: a solid cell takes, for each
 // direction q whose neighbour x + e_q is fluid, f_bb(q)(x + 2 e_q), in place
-// in the post-collision lattice.  Ghost cells are fluid (the ball never
-// reaches them).  A source is never a target: between two solid voxels of a
+// in the post-collision lattice.  The reference's mask covers the ghost
+// cells too: where the ball reaches the first ghost layer (ny or nz within r
+// of its centre), a ghost neighbour can be solid, and its mask value is
+// recomputed here.  A source is never a target: between two solid voxels of a
 // ball the midpoint voxel is solid too, so that direction is not taken.  The
 // (cell, q) pairs are therefore independent: thread t of row q = blockIdx.y
 // takes cell t of the box [lo, lo + n)^3 around the ball (clamped to the
@@ -577,7 +549,8 @@ __global__ void k_faithful_usr(const FusedParams P, int lo, int n)
     double v = *g.p;
     v = g.is_deep ? g.deep : v;
     const bool solid = valid & (mc == 1);
-    if (solid & (!in | (m != 1))) ((double *)P.src)[(uint64_t)q * P.pstride + c] = v;
+    const bool nb_solid = in ? m == 1 : faithful_sphere_solid(P.nx, xi, yi, zi);
+    if (solid & !nb_solid) ((double *)P.src)[(uint64_t)q * P.pstride + c] = v;
 }
 
 hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st)

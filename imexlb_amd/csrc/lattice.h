@@ -37,4 +37,34 @@ IMX_HD constexpr double w(int i) { return WVAL[WCLASS[i]]; }
 constexpr float H_SCALE[4] = {4.0f, 16.0f, 64.0f, 256.0f};
 constexpr float H_INV_SCALE[4] = {0.25f, 0.0625f, 0.015625f, 0.00390625f};
 
+// The d-th direction, in index order, of the nine with e_AXIS * SIGN > 0
+// (d < 9), and a direction's velocity and opposite, for kernels that give each
+// direction its own row of workgroups.
+template <int AXIS, int SIGN>
+IMX_HD inline int nth_dir(int d)
+{
+    int q = 0, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int e = AXIS == 0 ? EX[k] : AXIS == 1 ? EY[k] : EZ[k];
+        if (e * SIGN > 0) {
+            if (cnt == d) q = k;
+            ++cnt;
+        }
+    }
+    return q;
+}
+IMX_HD inline void dir_of(int q, int (&e)[3], int &bq)
+{
+    e[0] = e[1] = e[2] = bq = 0;
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        if (k == q) {
+            e[0] = EX[k];
+            e[1] = EY[k];
+            e[2] = EZ[k];
+            bq = BB[k];
+        }
+}
+
 }  // namespace imx

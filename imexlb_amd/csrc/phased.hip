@@ -221,10 +221,10 @@ __global__ void k_cavity_zero_planes(const Dims d, const Arrays a)
 // Faithful wall bounce-back on one face: for the cells (a, b) of the plane in
 // [G-1, l-G+1)^2 and every direction pointing into the domain,
 // f_q(plane) = f_bb(q)(src_plane, a + sa e_a, b + sb e_b).  Sources lie in
-// src_plane and targets in plane, so all loads are issued before the stores;
-// the axis and the inward sign are template arguments, which fixes the
-// directions taken at compile time (a runtime choice would put every load
-// under a branch that waits for it).
+// src_plane and targets in plane, so the (cell, q) pairs are independent:
+// thread t of row blockIdx.y takes cell t and the blockIdx.y-th such q (nine
+// times the lanes of a thread per cell hide the latency of the scattered
+// loads).  The axis and the inward sign are template arguments.
 template <int AXIS, int SIGN>
 __global__ void k_wall(const Dims d, const Arrays a, int plane, int src_plane, int sa, int sb)
 {
@@ -233,83 +233,61 @@ __global__ void k_wall(const Dims d, const Arrays a, int plane, int src_plane, i
     const int na = la - 2 * (G - 1), nb = lb - 2 * (G - 1);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na * nb) return;
+    const int q = nth_dir<AXIS, SIGN>(blockIdx.y);
+    int e[3], bq;
+    dir_of(q, e, bq);
     const int A = G - 1 + t % na, B = G - 1 + t / na;
-    double v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int e[3] = {EX[q], EY[q], EZ[q]};
-        if (e[AXIS] * SIGN <= 0) continue;
-        int si, sj, sk;
-        if (AXIS == 0) {
-            si = src_plane; sj = A + sa * e[1]; sk = B + sb * e[2];
-        } else if (AXIS == 1) {
-            si = A + sa * e[0]; sj = src_plane; sk = B + sb * e[2];
-        } else {
-            si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
-        }
-        v[q] = a.f[fidx(d, BB[q], si, sj, sk)];
+    int si, sj, sk, di, dj, dk;
+    if (AXIS == 0) {
+        si = src_plane; sj = A + sa * e[1]; sk = B + sb * e[2];
+        di = plane; dj = A; dk = B;
+    } else if (AXIS == 1) {
+        si = A + sa * e[0]; sj = src_plane; sk = B + sb * e[2];
+        di = A; dj = plane; dk = B;
+    } else {
+        si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
+        di = A; dj = B; dk = plane;
     }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int e[3] = {EX[q], EY[q], EZ[q]};
-        if (e[AXIS] * SIGN <= 0) continue;
-        int di, dj, dk;
-        if (AXIS == 0) {
-            di = plane; dj = A; dk = B;
-        } else if (AXIS == 1) {
-            di = A; dj = plane; dk = B;
-        } else {
-            di = A; dj = B; dk = plane;
-        }
-        a.f[fidx(d, q, di, dj, dk)] = v[q];
-    }
+    a.f[fidx(d, q, di, dj, dk)] = a.f[fidx(d, bq, si, sj, sk)];
 }
 
 // sphere/src/lbm.cpp:326-347.  Targets lie in the ghost plane, sources in
-// the first (inlet) or last (outlet) interior plane: loads first, then stores.
+// the first (inlet) or last (outlet) interior plane; a row of workgroups per
+// direction, as for the walls.
 template <int OUTLET>
 __global__ void k_sphere_inlet_outlet(const Dims d, const Arrays a)
 {
     const int na = d.ly - 2 * (G - 1), nb = d.lz - 2 * (G - 1);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= na * nb) return;
+    const int q = OUTLET ? nth_dir<0, -1>(blockIdx.y) : nth_dir<0, 1>(blockIdx.y);
+    int e[3], bq;
+    dir_of(q, e, bq);
     const int j = G - 1 + t % na, k = G - 1 + t / na;
     const int le0 = G + d.nx;
-    double v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (!OUTLET && EX[q] > 0) v[q] = a.f[fidx(d, q, G, j, k)];
-        if (OUTLET && EX[q] < 0) v[q] = a.f[fidx(d, q, le0 + EX[q], j + EY[q], k + EZ[q])];
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (!OUTLET && EX[q] > 0) a.f[fidx(d, q, G - 1, j, k)] = v[q];
-        if (OUTLET && EX[q] < 0) a.f[fidx(d, q, le0, j, k)] = v[q];
-    }
+    if (!OUTLET) a.f[fidx(d, q, G - 1, j, k)] = a.f[fidx(d, q, G, j, k)];
+    else a.f[fidx(d, q, le0, j, k)] = a.f[fidx(d, q, le0 + e[0], j + e[1], k + e[2])];
 }
 
-// "usrbc", sphere/src/lbm.cpp:350-356
-__global__ void k_sphere_usr(const Dims d, const Arrays a)
+// "usrbc", sphere/src/lbm.cpp:350-356, over the box [lo, lo + n)^3 of
+// interior cells around the ball: thread t of row q = blockIdx.y takes cell t
+// and direction q.  Every pair loads, the masks pick the store: no load waits
+// under a branch.  A source is never a target: between two solid voxels of a
+// ball the midpoint voxel is solid too.
+__global__ void k_sphere_usr(const Dims d, const Arrays a, int lo, int n)
 {
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= d.ncell) return;
-    int i, j, k;
-    decode(d, c, i, j, k);
-    if (!interior(d, i, j, k) || a.usr[c] != 0) return;
-    // every source (depth <= 2 from the interior) is loaded, the masks pick
-    // the stores: no load waits under a branch.  A source is never a target
-    // of this launch: between two solid voxels of a ball the midpoint voxel
-    // is solid too.
-    bool take[Q];
-    double v[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        take[q] = a.usr[cidx(d, i + EX[q], j + EY[q], k + EZ[q])] == 1;
-        v[q] = a.f[fidx(d, BB[q], i + 2 * EX[q], j + 2 * EY[q], k + 2 * EZ[q])];
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if (take[q]) a.f[fidx(d, q, i, j, k)] = v[q];
+    const int q = blockIdx.y;
+    int e[3], bq;
+    dir_of(q, e, bq);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n * n) return;
+    const int x = lo + t % n, y = lo + (t / n) % n, z = lo + t / (n * n);
+    if (x >= d.nx || y >= d.ny || z >= d.nz) return;
+    const int i = G + x, j = G + y, k = G + z;
+    const bool solid = a.usr[cidx(d, i, j, k)] == 0;
+    const bool take = a.usr[cidx(d, i + e[0], j + e[1], k + e[2])] == 1;
+    const double v = a.f[fidx(d, bq, i + 2 * e[0], j + 2 * e[1], k + 2 * e[2])];
+    if (solid & take) a.f[fidx(d, q, i, j, k)] = v;
 }
 
 // config-mode cavity: halfway bounce-back as a fill of the depth-1 ghost shell
@@ -819,7 +797,8 @@ static void wall(PhasedState *s, int axis, int plane, int src_plane, int sign, i
     const Dims &d = s->d;
     const int la = axis == 0 ? d.ly : d.lx, lb = axis == 2 ? d.ly : d.lz;
     const uint64_t n = (uint64_t)(la - 2 * (G - 1)) * (lb - 2 * (G - 1));
-    const dim3 g = grid_for(n), b(256);
+    dim3 g = grid_for(n), b(256);
+    g.y = 9;  // a row of workgroups per direction entering the domain
 #define IMX_WALL(AX, SG) hipLaunchKernelGGL((k_wall<AX, SG>), g, b, 0, s->st, d, s->a, plane, src_plane, sa, sb)
     if (axis == 0) {
         if (sign > 0) IMX_WALL(0, 1);
@@ -869,9 +848,16 @@ static void boundaries(PhasedState *s)
             wall(s, 2, ls - 1, ls + 1, +1, 2, 2);
             wall(s, 2, le2, le2 - 2, -1, 2, 2);
             const uint64_t n = (uint64_t)(d.ly - 2 * (G - 1)) * (d.lz - 2 * (G - 1));
-            hipLaunchKernelGGL(k_sphere_inlet_outlet<0>, grid_for(n), dim3(256), 0, s->st, d, s->a);
-            hipLaunchKernelGGL(k_sphere_inlet_outlet<1>, grid_for(n), dim3(256), 0, s->st, d, s->a);
-            hipLaunchKernelGGL(k_sphere_usr, grid_for(d.ncell), dim3(256), 0, s->st, d, s->a);
+            dim3 gio = grid_for(n);
+            gio.y = 9;
+            hipLaunchKernelGGL(k_sphere_inlet_outlet<0>, gio, dim3(256), 0, s->st, d, s->a);
+            hipLaunchKernelGGL(k_sphere_inlet_outlet<1>, gio, dim3(256), 0, s->st, d, s->a);
+            // the ball lies within [c - r, c + r] of its centre c on every axis
+            const int cc = (d.nx - 1) / 8, rr = (int)std::ceil((d.nx - 1) / 64.0);
+            const int lo = cc - rr < 0 ? 0 : cc - rr, nb = cc + rr - lo + 1;
+            dim3 gu = grid_for((uint64_t)nb * nb * nb);
+            gu.y = Q;
+            hipLaunchKernelGGL(k_sphere_usr, gu, dim3(256), 0, s->st, d, s->a, lo, nb);
         } else {
             hipLaunchKernelGGL(k_channel_planes, grid_for((uint64_t)d.ly * d.lz), dim3(256), 0, s->st, d, s->a,
                                s->feq_in);

@@ -98,6 +98,18 @@ def test_phased_engine_bitexact(case, mode):
         assert same(o.get_f(), g.get_f())
 
 
+@pytest.mark.parametrize("n", [(72, 9, 12), (130, 16, 17)])
+def test_phased_faithful_sphere_ball_in_ghost_layer(n):
+    """The phased engine where the reference's ball reaches the first ghost
+    layer: equal to the oracle bit for bit."""
+    o, g = make_pair(SPHERE, FAITHFUL, n, 0.08, 0.1, engine="phased")
+    for k in (1, 12):
+        o.run(k)
+        g.step(k)
+        moments_equal(o, g)
+        assert same(o.get_f(), g.get_f())
+
+
 @pytest.mark.parametrize("storage", ["pull", "aa"])
 def test_cavity_config_fused_bitexact(storage):
     n = (12, 10, 14)
@@ -424,12 +436,18 @@ def test_fused_faithful_cavity_bitexact(n):
     assert same(o.get_f(), g.get_f())
 
 
-@pytest.mark.parametrize("n", [(40, 14, 12), (24, 10, 9), (72, 20, 20), (9, 3, 4)])
+# the last four put solid cells of the ball (centre (nx - 1) / 8, radius
+# (nx - 1) / 64) into the first y or z ghost layer, next to solid interior cells
+SPHERE_GHOST_BALL = [(72, 9, 12), (72, 12, 9), (72, 8, 9), (130, 16, 17)]
+
+
+@pytest.mark.parametrize("n", [(40, 14, 12), (24, 10, 9), (72, 20, 20), (9, 3, 4)] + SPHERE_GHOST_BALL)
 def test_fused_faithful_sphere_bitexact(n):
     """The reference's own sphere rules on the fused engine: walls on y and z,
     the inlet and outlet ghost copies and usrbc fill the ghost ring (and the
     solid cells), one fused pass streams and applies usr, the parabolic inlet
-    and p = 0 at the outlet.  Equal to the oracle bit for bit."""
+    and p = 0 at the outlet.  Equal to the oracle bit for bit, also where the
+    ball reaches the ghost layer (usrbc then skips solid ghost neighbours)."""
     o, g = make_pair(SPHERE, FAITHFUL, n, 0.08, 0.1, engine="fused")
     moments_equal(o, g)
     for k in (1, 8, 31):
