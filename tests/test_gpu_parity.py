@@ -98,10 +98,11 @@ def test_phased_engine_bitexact(case, mode):
         assert same(o.get_f(), g.get_f())
 
 
-@pytest.mark.parametrize("n", [(72, 9, 12), (130, 16, 17)])
+@pytest.mark.parametrize("n", [(72, 9, 12), (130, 16, 17), (8, 3, 4)])
 def test_phased_faithful_sphere_ball_in_ghost_layer(n):
     """The phased engine where the reference's ball reaches the first ghost
-    layer: equal to the oracle bit for bit."""
+    layer, or (nx <= 8) lies on the inlet plane: equal to the oracle bit for
+    bit."""
     o, g = make_pair(SPHERE, FAITHFUL, n, 0.08, 0.1, engine="phased")
     for k in (1, 12):
         o.run(k)
@@ -436,12 +437,14 @@ def test_fused_faithful_cavity_bitexact(n):
     assert same(o.get_f(), g.get_f())
 
 
-# the last four put solid cells of the ball (centre (nx - 1) / 8, radius
-# (nx - 1) / 64) into the first y or z ghost layer, next to solid interior cells
+# these put solid cells of the ball (centre (nx - 1) / 8, radius (nx - 1) / 64)
+# into the first y or z ghost layer, next to solid interior cells
 SPHERE_GHOST_BALL = [(72, 9, 12), (72, 12, 9), (72, 8, 9), (130, 16, 17)]
 
 
-@pytest.mark.parametrize("n", [(40, 14, 12), (24, 10, 9), (72, 20, 20), (9, 3, 4)] + SPHERE_GHOST_BALL)
+# (8, 3, 4) and (6, 2, 2): nx <= 8 puts the one-voxel ball on the inlet plane
+@pytest.mark.parametrize("n", [(40, 14, 12), (24, 10, 9), (72, 20, 20), (9, 3, 4), (8, 3, 4), (6, 2, 2)] +
+                         SPHERE_GHOST_BALL)
 def test_fused_faithful_sphere_bitexact(n):
     """The reference's own sphere rules on the fused engine: walls on y and z,
     the inlet and outlet ghost copies and usrbc fill the ghost ring (and the
@@ -491,12 +494,18 @@ def test_fused_faithful_cavity_six_calls_and_launch_shapes(tmp_path):
 def test_fused_faithful_placement_search_keeps_results(case, n):
     """The placement search re-allocates the lattices of a faithful run (the
     ghost ring stays) and re-initialises it: the run then equals the
-    oracle's, bit for bit."""
+    oracle's, bit for bit, also through a replayed step graph (whose ring
+    offsets follow the kept lattices)."""
     o, g = make_pair(case, FAITHFUL, n, 0.08, 0.1, engine="fused")
     kept, ms = g.tune_placement(3, steps=2)
     assert 0 <= kept < 3 and ms[0] > 0
     o.run(9)
     g.step(9)
+    moments_equal(o, g)
+    o.run(20)
+    g.step(20, moments_every=0)
+    g.step(1)
+    o.run(1)
     moments_equal(o, g)
     o.Collision()
     assert same(o.get_f(), g.get_f())
