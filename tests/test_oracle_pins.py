@@ -186,6 +186,10 @@ CROSS = [
     (LDC, FAITHFUL, (8, 7, 9), dict(tau0=0.1, u0=0.1)),
     (LDC, CONFIG, (8, 7, 9), dict(tau0=0.1, u0=0.0, lid_u=0.1)),
     (SPHERE, FAITHFUL, (40, 12, 12), dict(tau0=0.1, u0=0.1)),
+    # the ball reaches the first y ghost layer (solid ghost neighbours of solid cells)
+    (SPHERE, FAITHFUL, (72, 9, 12), dict(tau0=0.1, u0=0.1)),
+    # nx <= 8: the one-voxel ball lies on the inlet plane
+    (SPHERE, FAITHFUL, (8, 3, 4), dict(tau0=0.1, u0=0.1)),
     (SPHERE, CONFIG, (24, 12, 14), dict(tau0=0.1, u0=0.05, sphere=(8.0, 6.0, 7.0, 3.0))),
 ]
 
