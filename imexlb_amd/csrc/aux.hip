@@ -460,7 +460,7 @@ __global__ void k_faithful_unpack(const FusedParams P)
 // (a row per direction: the pairs are independent, and nine times the lanes
 // hide the latency of the scattered loads).
 template <int AXIS, int SIGN>
-__global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, int sa, int sb)
+__device__ __forceinline__ void faithful_wall(const FusedParams &P, int plane, int src_plane, int sa, int sb)
 {
     const int la = AXIS == 0 ? P.ny : P.nx, lb = AXIS == 2 ? P.ny : P.nz;
     const int na = la + 2, nb = lb + 2;
@@ -486,6 +486,20 @@ __global__ void k_faithful_wall(const FusedParams P, int plane, int src_plane, i
     v = g.is_deep ? g.deep : v;
     const uint64_t r = (uint64_t)ring_index(P, di, dj, dk);
     P.ring[(uint64_t)q * P.ring_n + r] = v;
+}
+
+// The two walls of one axis in one launch: blockIdx.z = 0 the low wall
+// (plane -1, sources on plane 1), 1 the high wall (plane n, sources on plane
+// n - 2; its b-stride sb_high).  With n >= 2 both source planes lie inside the
+// domain on this axis, so neither wall reads what the other writes and their
+// order does not matter; the axes still follow the reference's order, since
+// a later axis reads edge cells an earlier one wrote.
+template <int AXIS>
+__global__ void k_faithful_walls(const FusedParams P, int sb_high)
+{
+    const int n = AXIS == 0 ? P.nx : AXIS == 1 ? P.ny : P.nz;
+    if (blockIdx.z == 0) faithful_wall<AXIS, 1>(P, -1, 1, 2, 2);
+    else faithful_wall<AXIS, -1>(P, n, n - 2, 2, sb_high);
 }
 
 // sphere/src/lbm.cpp:326-347: the inlet ghost plane takes the first
@@ -558,13 +572,13 @@ hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st)
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     // nine rows of workgroups: one per direction entering the domain
     auto grid = [](int la, int lb) { return dim3((unsigned)(((la + 2) * (lb + 2) + 255) / 256), 9); };
+    // the same, times the two walls of an axis
+    auto grid2 = [](int la, int lb) { return dim3((unsigned)(((la + 2) * (lb + 2) + 255) / 256), 9, 2); };
     if (P.faithful_sphere) {
         // bcf bcb bcd bcu, inlet, outlet, usrbc (sphere/src/lbm.cpp:277-356);
         // one rank: the unpack has nothing to write
-        hipLaunchKernelGGL((k_faithful_wall<1, 1>), grid(nx, nz), dim3(256), 0, st, P, -1, 1, 2, 2);
-        hipLaunchKernelGGL((k_faithful_wall<1, -1>), grid(nx, nz), dim3(256), 0, st, P, ny, ny - 2, 2, 2);
-        hipLaunchKernelGGL((k_faithful_wall<2, 1>), grid(nx, ny), dim3(256), 0, st, P, -1, 1, 2, 2);
-        hipLaunchKernelGGL((k_faithful_wall<2, -1>), grid(nx, ny), dim3(256), 0, st, P, nz, nz - 2, 2, 2);
+        hipLaunchKernelGGL(k_faithful_walls<1>, grid2(nx, nz), dim3(256), 0, st, P, 2);
+        hipLaunchKernelGGL(k_faithful_walls<2>, grid2(nx, ny), dim3(256), 0, st, P, 2);
         hipLaunchKernelGGL(k_faithful_inlet_outlet<0>, grid(ny, nz), dim3(256), 0, st, P);
         hipLaunchKernelGGL(k_faithful_inlet_outlet<1>, grid(ny, nz), dim3(256), 0, st, P);
         // the ball lies within [c - r, c + r] of its centre c on every axis
@@ -575,13 +589,10 @@ hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st)
     }
     const uint32_t nun = 2u * (P.ny + 2) * (P.nz + 2) + (uint32_t)P.nx * (P.nz + 2) + (uint32_t)P.nx * (P.ny + 2);
     hipLaunchKernelGGL(k_faithful_unpack, dim3((nun + 255) / 256), dim3(256), 0, st, P);
-    // bcl bcr bcf bcb bcd bcu; the top wall reads j + e_y (:298)
-    hipLaunchKernelGGL((k_faithful_wall<0, 1>), grid(ny, nz), dim3(256), 0, st, P, -1, 1, 2, 2);
-    hipLaunchKernelGGL((k_faithful_wall<0, -1>), grid(ny, nz), dim3(256), 0, st, P, nx, nx - 2, 2, 2);
-    hipLaunchKernelGGL((k_faithful_wall<1, 1>), grid(nx, nz), dim3(256), 0, st, P, -1, 1, 2, 2);
-    hipLaunchKernelGGL((k_faithful_wall<1, -1>), grid(nx, nz), dim3(256), 0, st, P, ny, ny - 2, 2, 2);
-    hipLaunchKernelGGL((k_faithful_wall<2, 1>), grid(nx, ny), dim3(256), 0, st, P, -1, 1, 2, 2);
-    hipLaunchKernelGGL((k_faithful_wall<2, -1>), grid(nx, ny), dim3(256), 0, st, P, nz, nz - 2, 2, 1);
+    // bcl bcr, bcf bcb, bcd bcu; the top wall reads j + e_y (:298)
+    hipLaunchKernelGGL(k_faithful_walls<0>, grid2(ny, nz), dim3(256), 0, st, P, 2);
+    hipLaunchKernelGGL(k_faithful_walls<1>, grid2(nx, nz), dim3(256), 0, st, P, 2);
+    hipLaunchKernelGGL(k_faithful_walls<2>, grid2(nx, ny), dim3(256), 0, st, P, 1);
     return hipGetLastError();
 }
 
