@@ -53,6 +53,29 @@ def feq(p, u, v, w):
     return out
 
 
+def lid_term(q, rho0, lid_u):
+    """Momentum carried into the fluid by halfway bounce-back off the moving
+    lid (config-mode cavity; the reference's own lid is an overwrite in
+    Update, lid_driven_cavity/src/lbm.cpp:343-344): 2 w_q rho0 (e_q . u_w) / cs^2
+    for the direction q leaving the lid, cs^2 = 1/3, lid velocity (lid_u, 0, 0)."""
+    return 2.0 * W[q] * rho0 * float(E[q] @ np.array([lid_u, 0.0, 0.0])) * 3.0
+
+
+def sphere_force(f, usr):
+    """Momentum-exchange drag proxy, enumerated from the solid side: every
+    solid cell s and direction q whose upstream cell s - e_q is fluid and
+    inside the box hands the body 2 f*_q(s - e_q) e_q.  f is post-collision,
+    [q, x, y, z] over the interior; usr is 1 for fluid, 0 for solid."""
+    nx, ny, nz = usr.shape
+    out = np.zeros(3)
+    for s in np.argwhere(usr == 0):
+        for q in range(1, Q):
+            x, y, z = s - E[q]
+            if 0 <= x < nx and 0 <= y < ny and 0 <= z < nz and usr[x, y, z] == 1:
+                out += 2.0 * f[q, x, y, z] * E[q]
+    return out
+
+
 def moments(f):
     p = np.zeros(f.shape[1:])
     u = np.zeros_like(p)
@@ -180,7 +203,7 @@ class NpModel:
         for q in range(Q):
             sx, sy, sz = X - E[q, 0], Y - E[q, 1], Z - E[q, 2]
             out = (sx < 0) | (sx >= nx) | (sy < 0) | (sy >= ny) | (sz < 0) | (sz >= nz)
-            corr = 6.0 * W[q] * self.rho0 * (E[q, 0] * self.lid_u)
+            corr = lid_term(q, self.rho0, self.lid_u)
             bounce = np.where(sz >= nz, fpost[BB[q]] + corr, fpost[BB[q]])
             pulled[q] = np.where(out, bounce, pulled[q])
         self.f = pulled

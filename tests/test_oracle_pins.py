@@ -298,3 +298,150 @@ def test_sphere_faithful_mask_matches_reference_formula():
     want = np.where((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2 <= r * r, 0, 1)
     assert np.array_equal(usr, want)
     assert (usr == 0).sum() > 0
+
+
+# ---------------------------------------------------------------- closed forms of the config-mode terms
+#
+# Neither term exists in the reference: its lid is an overwrite of u in Update
+# (lid_driven_cavity/src/lbm.cpp:343-344) and its sphere has no drag
+# (sphere/src/lbm.cpp:350-356).  Both are pinned here by what the mathematics
+# fixes, on the C oracle and on the NumPy restatement, and each check is shown
+# to fail on a planted mistake.
+
+def _lid_one_step(moms, rho0, lid_u):
+    """One config-cavity step from rest.  A top-layer cell away from the side
+    walls pulls its nine e_z = -1 populations off the lid:
+    f_i = w_i rho0 + 6 w_i rho0 e_ix u_lid, so u_x = 6 rho0 u_lid sum w_i e_ix^2
+    = rho0 u_lid / 3 (the sum over e_z = -1 is 2/54 + 4/216 = 1/18), and
+    p = sum f / 3 = rho0 / 3 (sum w_i e_ix = 0 there).  Every cell below the
+    top layer is still at rest."""
+    p, u, v, w = moms  # [z][y][x]
+    top = (-1, slice(1, -1), slice(1, -1))
+    ok = (np.abs(u[top] - rho0 * lid_u / 3.0).max() <= 1e-15
+          and np.abs(p[top] - rho0 / 3.0).max() <= 1e-15
+          and np.abs(v[top]).max() <= 1e-16 and np.abs(w[top]).max() <= 1e-16)
+    below = (slice(0, -1),)
+    ok = ok and all(np.abs(a[below]).max() <= 1e-16 for a in (u, v, w))
+    return ok and np.abs(p[below] - rho0 / 3.0).max() <= 1e-15
+
+
+LID = dict(tau0=0.1, rho0=1.25, u0=0.0, lid_u=0.08)
+
+
+def test_config_lid_term_one_step_closed_form():
+    n = (7, 6, 5)
+    o = Oracle(LDC, *n, mode=CONFIG, **LID)
+    o.Initialize()
+    o.run(1)
+    assert _lid_one_step(o.moments(), LID["rho0"], LID["lid_u"])
+    m = oracle_np.NpModel(LDC, *n, mode=CONFIG, **LID)
+    m.initialize()
+    m.run(1)
+    assert _lid_one_step(m.moments(), LID["rho0"], LID["lid_u"])
+    # the lid's momentum reaches only the top layer in one step
+    assert abs(o.moments()[1][-1, 2, 3] - LID["rho0"] * LID["lid_u"] / 3.0) <= 1e-15
+
+
+def test_lid_closed_form_catches_planted_mistakes(monkeypatch):
+    """A flipped lid sign, a missing 1/cs^2, or the lid velocity taken along y
+    in the restatement's lid term each break the one-step closed form."""
+    n = (7, 6, 5)
+    orig = oracle_np.lid_term
+    bad_terms = [
+        lambda q, r, u: -orig(q, r, u),
+        lambda q, r, u: orig(q, r, u) / 3.0,
+        lambda q, r, u: 6.0 * oracle_np.W[q] * r * oracle_np.E[q, 1] * u,
+    ]
+    for bad in bad_terms:
+        monkeypatch.setattr(oracle_np, "lid_term", bad)
+        m = oracle_np.NpModel(LDC, *n, mode=CONFIG, **LID)
+        m.initialize()
+        m.run(1)
+        assert not _lid_one_step(m.moments(), LID["rho0"], LID["lid_u"])
+
+
+SPH = dict(tau0=0.1, rho0=1.0, u0=0.05)
+SPH_N = (9, 8, 7)
+SPH_BALL = (4.0, 3.0, 3.0, 0.4)  # one solid voxel, at (4, 3, 3)
+
+
+def _np_equilibrium_force(vel):
+    """Drag proxy of the NumPy restatement on a uniform equilibrium field of
+    velocity vel around the one-voxel ball."""
+    m = oracle_np.NpModel(SPHERE, *SPH_N, mode=CONFIG, sphere=SPH_BALL, **SPH)
+    m.initialize()
+    assert (m.usr == 0).sum() == 1
+    f = oracle_np.feq(np.full(SPH_N, SPH["rho0"] / 3.0), *(np.full(SPH_N, c) for c in vel))
+    return oracle_np.sphere_force(f, m.usr)
+
+
+def test_sphere_force_closed_form():
+    """A one-voxel solid in a uniform equilibrium stream of velocity U: each of
+    its 26 fluid neighbours s - e_i hands it 2 feq_i(U) e_i, and
+    sum_i feq_i e_i = U (isotropy of the weights), so F = 2 U."""
+    u0 = SPH["u0"]
+    o = Oracle(SPHERE, *SPH_N, mode=CONFIG, sphere=SPH_BALL, **SPH)
+    o.Initialize()
+    assert (o.fields_full()[4][G:-G, G:-G, G:-G] == 0).sum() == 1
+    o.Collision()  # the identity at equilibrium
+    assert np.abs(o.sphere_force() - np.array([2 * u0, 0.0, 0.0])).max() <= 1e-15
+    # any uniform equilibrium velocity, set directly as post-collision populations
+    vel = (0.031, -0.047, 0.019)
+    f = oracle_np.feq(np.full((o.lx, o.ly, o.lz), SPH["rho0"] / 3.0),
+                      *(np.full((o.lx, o.ly, o.lz), c) for c in vel))
+    o.set_f_full(f)
+    assert np.abs(o.sphere_force() - 2 * np.array(vel)).max() <= 1e-15
+    assert np.abs(_np_equilibrium_force((u0, 0.0, 0.0)) - np.array([2 * u0, 0.0, 0.0])).max() <= 1e-15
+    assert np.abs(_np_equilibrium_force(vel) - 2 * np.array(vel)).max() <= 1e-15
+
+
+def test_sphere_force_random_field_matches_restatement():
+    """On random post-collision populations around a ball of several voxels the
+    C oracle (enumerated from the fluid side) equals the restatement
+    (enumerated from the solid side)."""
+    rng = np.random.default_rng(11)
+    ball = (6.0, 5.0, 5.0, 1.8)
+    n = (13, 11, 11)
+    o = Oracle(SPHERE, *n, mode=CONFIG, sphere=ball, **SPH)
+    o.Initialize()
+    f = rng.standard_normal((27, o.lx, o.ly, o.lz))
+    o.set_f_full(f)
+    m = oracle_np.NpModel(SPHERE, *n, mode=CONFIG, sphere=ball, **SPH)
+    m.initialize()
+    assert (m.usr == 0).sum() > 10
+    want = oracle_np.sphere_force(f[:, G:-G, G:-G, G:-G], m.usr)
+    assert np.abs(o.sphere_force() - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_sphere_force_closed_form_catches_planted_mistakes(monkeypatch):
+    """A missing factor 2 or a reversed link direction in the restatement's
+    drag proxy breaks F = 2 U."""
+    vel = (0.031, -0.047, 0.019)
+    orig = oracle_np.sphere_force
+    for bad in (lambda f, usr: orig(f, usr) / 2.0, lambda f, usr: -orig(f, usr)):
+        monkeypatch.setattr(oracle_np, "sphere_force", bad)
+        got = _np_equilibrium_force(vel)
+        assert not np.abs(got - 2 * np.array(vel)).max() <= 1e-15
+
+
+def test_sphere_force_cross_check_catches_upstream_downstream_swap():
+    """Taking the populations of the solid's downstream neighbours s + e_i
+    leaves a uniform field's F = 2 U unchanged, so the random-field
+    cross-check is what catches it."""
+    rng = np.random.default_rng(11)
+    ball = (6.0, 5.0, 5.0, 1.8)
+    n = (13, 11, 11)
+    o = Oracle(SPHERE, *n, mode=CONFIG, sphere=ball, **SPH)
+    o.Initialize()
+    f = rng.standard_normal((27, o.lx, o.ly, o.lz))
+    o.set_f_full(f)
+    m = oracle_np.NpModel(SPHERE, *n, mode=CONFIG, sphere=ball, **SPH)
+    m.initialize()
+    fi, usr = f[:, G:-G, G:-G, G:-G], m.usr
+    bad = np.zeros(3)
+    for s_ in np.argwhere(usr == 0):
+        for q in range(1, 27):
+            x, y, z = s_ + oracle_np.E[q]
+            if usr[x, y, z] == 1:
+                bad += 2.0 * fi[q, x, y, z] * oracle_np.E[q]
+    assert not np.abs(o.sphere_force() - bad).max() <= 1e-12 * np.abs(bad).max()
