@@ -94,6 +94,7 @@ SIGNATURES = [
     ("imexlb_last_error", ctypes.c_char_p, [_vp]),
     ("imexlb_strerror", ctypes.c_char_p, [_i]),
     ("imexlb_copy_bandwidth", _i, [_i, ctypes.c_size_t, _i, _dp, _fp]),
+    ("imexlb_bandwidth_probe", _i, [_i, _i, ctypes.c_size_t, _i, _dp, _fp]),
     ("imexlb_system_read", _i, [ctypes.c_char_p, _i, _i, ctypes.POINTER(SystemC)]),
     ("imexlb_comm_layout", _i, [_i, _i, _i, _i, _i, _i, ctypes.POINTER(Comm)]),
     ("imexlb_checkpoint_write", _i, [_vp, ctypes.c_char_p]),
@@ -110,6 +111,7 @@ SIGNATURES = [
     ("imexlb_ensemble_step_timed", _i, [_vp, _i, _i, _i, _fp]),
     ("imexlb_ensemble_get_moments", _i, [_vp, _i, _dp, _dp, _dp, _dp]),
     ("imexlb_ensemble_info", _i, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    ("imexlb_ensemble_status", _i, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_ensemble_set_launch", _i, [_vp, _i, _i, _i]),
     ("imexlb_ensemble_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_ensemble_tune_placement", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float),

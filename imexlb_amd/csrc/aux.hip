@@ -3,6 +3,8 @@
 // state is the oracle's, bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fused.h"
 #include "kernels.h"
 
@@ -317,6 +319,38 @@ hipError_t launch_force(int dtype, const FusedParams &P, double *partials, int n
     return hipGetLastError();
 }
 
+// Domain health: *flag stays 1 while every value of the four moment fields
+// is finite and is set to 0 by any lane that finds one that is not (all
+// writers store the same 0).  A moment sums all 27 populations of its cell,
+// so a non-finite population shows in the moments of the same step.
+__global__ void __launch_bounds__(256) k_nonfinite(const double *p, const double *u, const double *v,
+                                                   const double *w, uint32_t n, int32_t *flag)
+{
+    bool bad = false;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        bad |= !(isfinite(p[c]) && isfinite(u[c]) && isfinite(v[c]) && isfinite(w[c]));
+    if (bad) *flag = 0;
+}
+
+__global__ void k_fill_i32(int32_t *a, int n, int32_t v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+hipError_t launch_finite_flags(const double *const *mom, const uint32_t *ncell, int ndom, int32_t *flags,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3((ndom + 255) / 256), dim3(256), 0, st, flags, ndom, 1);
+    for (int d = 0; d < ndom; ++d) {
+        const uint32_t n = ncell[d];
+        const int grid = (int)std::min<uint32_t>(1024u, (n + 255) / 256);
+        hipLaunchKernelGGL(k_nonfinite, dim3(grid), dim3(256), 0, st, mom[4 * d], mom[4 * d + 1], mom[4 * d + 2],
+                           mom[4 * d + 3], n, flags + d);
+    }
+    return hipGetLastError();
+}
+
 // Plain copy, one 16-byte vector per lane, one pass: the reference point for
 // the hardware counters and the measured bandwidth ceiling.
 typedef double dvec2 __attribute__((ext_vector_type(2)));
@@ -332,6 +366,105 @@ hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st)
     const size_t n2 = n / 2;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, (const dvec2 *)src,
                        (dvec2 *)dst, n2);
+    return hipGetLastError();
+}
+
+// Bandwidth probes: copy, read-only and write-only streams with V 16-byte
+// vectors per lane (a workgroup covers V * 256 consecutive vectors, each
+// sub-pass coalesced), optionally non-temporal, or a grid-stride loop over a
+// fixed grid.  They bound what any kernel with one read and one write stream
+// per value can reach on the part.
+template <int V, bool NT>
+__global__ void __launch_bounds__(256) k_copy_v(const dvec2 *__restrict__ src, dvec2 *__restrict__ dst, size_t n2)
+{
+    const size_t base = (size_t)blockIdx.x * 256 * V + threadIdx.x;
+    dvec2 x[V];
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+        const size_t i = base + (size_t)r * 256;
+        if (i < n2) x[r] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+    }
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+        const size_t i = base + (size_t)r * 256;
+        if (i < n2) {
+            if (NT) __builtin_nontemporal_store(x[r], dst + i);
+            else dst[i] = x[r];
+        }
+    }
+}
+
+template <bool NT>
+__global__ void __launch_bounds__(256) k_copy_gs(const dvec2 *__restrict__ src, dvec2 *__restrict__ dst, size_t n2)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += (size_t)gridDim.x * 256) {
+        if (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+        else dst[i] = src[i];
+    }
+}
+
+// Read-only: the loaded values feed a sum that is stored only if it equals a
+// value the data never sums to, so no load can be dropped.
+template <int V, bool NT>
+__global__ void __launch_bounds__(256) k_read_v(const dvec2 *__restrict__ src, size_t n2, double never,
+                                                double *__restrict__ sink)
+{
+    const size_t base = (size_t)blockIdx.x * 256 * V + threadIdx.x;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+        const size_t i = base + (size_t)r * 256;
+        if (i < n2) {
+            const dvec2 x = NT ? __builtin_nontemporal_load(src + i) : src[i];
+            acc += x.x + x.y;
+        }
+    }
+    if (acc == never) sink[threadIdx.x] = acc;
+}
+
+template <int V, bool NT>
+__global__ void __launch_bounds__(256) k_write_v(dvec2 *__restrict__ dst, size_t n2, double v)
+{
+    const size_t base = (size_t)blockIdx.x * 256 * V + threadIdx.x;
+    const dvec2 x = {v, v};
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+        const size_t i = base + (size_t)r * 256;
+        if (i < n2) {
+            if (NT) __builtin_nontemporal_store(x, dst + i);
+            else dst[i] = x;
+        }
+    }
+}
+
+int bandwidth_probe_streams(int kind)
+{
+    switch (kind) {
+    case 0: case 1: case 2: case 3: case 4: case 5: return 2;
+    case 6: case 7: case 8: case 9: return 1;
+    default: return 0;
+    }
+}
+
+hipError_t launch_bandwidth_probe(int kind, const double *src, double *dst, size_t n, int cus, hipStream_t st)
+{
+    const size_t n2 = n / 2;
+    const auto grid = [&](int v) { return dim3((unsigned)((n2 + 256 * (size_t)v - 1) / (256 * (size_t)v))); };
+    const dvec2 *s2 = (const dvec2 *)src;
+    dvec2 *d2 = (dvec2 *)dst;
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((k_copy_v<1, false>), grid(1), dim3(256), 0, st, s2, d2, n2); break;
+    case 1: hipLaunchKernelGGL((k_copy_v<1, true>), grid(1), dim3(256), 0, st, s2, d2, n2); break;
+    case 2: hipLaunchKernelGGL((k_copy_v<4, false>), grid(4), dim3(256), 0, st, s2, d2, n2); break;
+    case 3: hipLaunchKernelGGL((k_copy_v<4, true>), grid(4), dim3(256), 0, st, s2, d2, n2); break;
+    case 4: hipLaunchKernelGGL((k_copy_gs<false>), dim3(cus * 8), dim3(256), 0, st, s2, d2, n2); break;
+    case 5: hipLaunchKernelGGL((k_copy_gs<true>), dim3(cus * 8), dim3(256), 0, st, s2, d2, n2); break;
+    case 6: hipLaunchKernelGGL((k_read_v<4, false>), grid(4), dim3(256), 0, st, s2, n2, -1.0, dst); break;
+    case 7: hipLaunchKernelGGL((k_read_v<4, true>), grid(4), dim3(256), 0, st, s2, n2, -1.0, dst); break;
+    case 8: hipLaunchKernelGGL((k_write_v<4, false>), grid(4), dim3(256), 0, st, d2, n2, 0.5); break;
+    case 9: hipLaunchKernelGGL((k_write_v<4, true>), grid(4), dim3(256), 0, st, d2, n2, 0.5); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -818,11 +818,28 @@ int imexlb_checkpoint_read(imexlb_handle *h, const char *path)
         std::fclose(fp);
         return fail(h, IMEXLB_EINVAL, "checkpoint %s is for another case or extent", path);
     }
-    // the fused faithful sphere recomputes its deeper ghost cells from the
-    // initial fields on every step, which depend on u0 and the seed
-    if (ok && h->ring && h->cfg.case_id == IMEXLB_CASE_SPHERE && (c.u0 != h->cfg.u0 || c.seed != h->cfg.seed)) {
-        std::fclose(fp);
-        return fail(h, IMEXLB_EINVAL, "checkpoint %s is for another u0 or seed", path);
+    // the physics the populations were advanced with: the BGK rate and the
+    // density everywhere; the lid of the config cavity; the inlet speed, the
+    // config ball (its mask must match the populations) and the faithful
+    // sphere's seed (its deeper ghost cells are recomputed from the initial
+    // fields on every step).  The TGV amplitude only sets the initial field.
+    if (ok) {
+        const char *what = nullptr;
+        if (c.tau0 != h->cfg.tau0) what = "tau0";
+        else if (c.rho0 != h->cfg.rho0) what = "rho0";
+        else if (c.case_id == IMEXLB_CASE_CAVITY && c.mode == IMEXLB_MODE_CONFIG && c.lid_u != h->cfg.lid_u)
+            what = "lid speed";
+        else if (c.case_id == IMEXLB_CASE_SPHERE && c.u0 != h->cfg.u0) what = "inlet speed u0";
+        else if (c.case_id == IMEXLB_CASE_SPHERE && c.mode == IMEXLB_MODE_CONFIG &&
+                 (c.sphere_cx != h->cfg.sphere_cx || c.sphere_cy != h->cfg.sphere_cy ||
+                  c.sphere_cz != h->cfg.sphere_cz || c.sphere_r != h->cfg.sphere_r))
+            what = "sphere centre or radius";
+        else if (c.case_id == IMEXLB_CASE_SPHERE && c.mode == IMEXLB_MODE_FAITHFUL && c.seed != h->cfg.seed)
+            what = "seed";
+        if (what) {
+            std::fclose(fp);
+            return fail(h, IMEXLB_EINVAL, "checkpoint %s was written with another %s", path, what);
+        }
     }
     ok = ok && steps >= 0 && std::fread(f.data(), 8, n, fp) == n;
     std::fclose(fp);
@@ -1006,10 +1023,16 @@ int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per
     return rc2;
 }
 
-int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch)
+// Times `reps` launches of bandwidth probe `kind` (-1: the plain copy of
+// launch_copy) over `bytes` bytes per stream.
+static int time_stream_kernel(int device, int kind, size_t bytes, int reps, double *gbps, float *ms_per_launch)
 {
-    if (bytes < 1024 || reps < 1 || !gbps || !ms_per_launch) return IMEXLB_EINVAL;
+    if (bytes < 4096 || reps < 1 || !gbps || !ms_per_launch) return IMEXLB_EINVAL;
+    const int streams = kind < 0 ? 2 : bandwidth_probe_streams(kind);
+    if (streams == 0) return IMEXLB_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return IMEXLB_ENODEV;
+    int cus = 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
     const size_t n = bytes / 8 / 2 * 2;
     double *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, n * 8) != hipSuccess) return IMEXLB_ENOMEM;
@@ -1017,30 +1040,46 @@ int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, floa
         hipFree(a);
         return IMEXLB_ENOMEM;
     }
-    hipStream_t st;
-    hipEvent_t e0, e1;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = IMEXLB_OK;
-    if (hipStreamCreate(&st) != hipSuccess) rc = IMEXLB_EHIP;
-    if (!rc && (hipMemsetAsync(a, 0, n * 8, st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-                hipEventCreate(&e1) != hipSuccess))
+    if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = IMEXLB_EHIP;
+    if (!rc && (hipMemsetAsync(a, 0, n * 8, st) != hipSuccess || hipMemsetAsync(b, 0, n * 8, st) != hipSuccess))
+        rc = IMEXLB_EHIP;
+    auto launch = [&]() { return kind < 0 ? launch_copy(a, b, n, st) : launch_bandwidth_probe(kind, a, b, n, cus, st); };
+    if (!rc && launch() != hipSuccess) rc = IMEXLB_EHIP;  // warm-up
     if (!rc) {
-        launch_copy(a, b, n, st);  // warm-up
         hipEventRecord(e0, st);
-        for (int r = 0; r < reps; ++r) launch_copy(a, b, n, st);
+        for (int r = 0; r < reps && !rc; ++r)
+            if (launch() != hipSuccess) rc = IMEXLB_EHIP;
         hipEventRecord(e1, st);
         if (hipEventSynchronize(e1) != hipSuccess) rc = IMEXLB_EHIP;
         float ms = 0;
         hipEventElapsedTime(&ms, e0, e1);
         *ms_per_launch = ms / reps;
-        *gbps = 2.0 * (double)n * 8.0 / (*ms_per_launch * 1e-3) / 1e9;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        *gbps = (double)streams * (double)n * 8.0 / (*ms_per_launch * 1e-3) / 1e9;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) {
+        hipStreamSynchronize(st);
         hipStreamDestroy(st);
     }
     hipFree(a);
     hipFree(b);
     return rc;
+}
+
+int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch)
+{
+    return time_stream_kernel(device, -1, bytes, reps, gbps, ms_per_launch);
+}
+
+int imexlb_bandwidth_probe(int device, int kind, size_t bytes, int reps, double *gbps, float *ms_per_launch)
+{
+    if (kind < 0) return IMEXLB_EINVAL;
+    return time_stream_kernel(device, kind, bytes, reps, gbps, ms_per_launch);
 }
 
 }  // extern "C"

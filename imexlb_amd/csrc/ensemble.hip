@@ -122,6 +122,7 @@ struct imexlb_ensemble {
     std::vector<void *> lat;             // 2 per domain (second null for AA)
     std::vector<double *> mom;           // 4 per domain
     std::vector<double *> tabs;          // TGV init tables per domain
+    int32_t *dflags = nullptr;           // per-domain finite flags (imexlb_ensemble_status)
     double *dummy = nullptr;
     int block = 256, cpt = 1, grid = 0, cus = 256;
     int64_t steps = 0, mom_step = -1, cells = 0, device_bytes = 0;
@@ -314,6 +315,7 @@ int imexlb_ensemble_destroy(imexlb_ensemble *e)
         if (p) (void)hipFree(p);
     if (e->dp) (void)hipFree(e->dp);
     if (e->dtiles) (void)hipFree(e->dtiles);
+    if (e->dflags) (void)hipFree(e->dflags);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return IMEXLB_OK;
@@ -424,6 +426,7 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
         }
     }
     if (hipMalloc(&e->dp, e->hp.size() * sizeof(FusedParams)) != hipSuccess) return bail(IMEXLB_ENOMEM);
+    if (hipMalloc(&e->dflags, (size_t)n * sizeof(int32_t)) != hipSuccess) return bail(IMEXLB_ENOMEM);
     build_tiles(e);
     if (upload_tables(e)) return bail(IMEXLB_EHIP);
     *out = e;
@@ -529,6 +532,29 @@ int imexlb_ensemble_get_moments(imexlb_ensemble *e, int d, double *p, double *u,
     for (int k = 0; k < 4; ++k)
         if (out[k]) ECHK(e, hipMemcpyAsync(out[k], e->mom[4 * d + k], nb, hipMemcpyDeviceToHost, e->stream));
     ECHK(e, hipStreamSynchronize(e->stream));
+    return IMEXLB_OK;
+}
+
+// finite[d] = 1 while domain d's moments of the current step are all finite,
+// 0 once the domain has blown up; *unstable = the number of zeros.
+int imexlb_ensemble_status(imexlb_ensemble *e, int32_t *finite, int32_t *unstable)
+{
+    if (!e) return IMEXLB_EINVAL;
+    if (e->mom_step != e->steps)
+        return efail(e, IMEXLB_ESTATE, "status needs the moments of the current step %lld", (long long)e->steps);
+    ECHK(e, hipSetDevice(e->device));
+    std::vector<uint32_t> nc(e->n);
+    for (int d = 0; d < e->n; ++d) nc[d] = e->hp[2 * d].ncell;
+    ECHK(e, launch_finite_flags(e->mom.data(), nc.data(), e->n, e->dflags, e->stream));
+    std::vector<int32_t> fl(e->n);
+    ECHK(e, hipMemcpyAsync(fl.data(), e->dflags, fl.size() * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+    ECHK(e, hipStreamSynchronize(e->stream));
+    int32_t bad = 0;
+    for (int d = 0; d < e->n; ++d) {
+        bad += fl[d] == 0;
+        if (finite) finite[d] = fl[d];
+    }
+    if (unstable) *unstable = bad;
     return IMEXLB_OK;
 }
 

@@ -170,6 +170,16 @@ hipError_t launch_force(int dtype, const FusedParams &P, double *partials, int n
                         double *out3, hipStream_t st);
 
 hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st);
+// Bandwidth probe `kind` (imexlb_bandwidth_probe) over n doubles; returns the
+// number of streams it moves (2 for copies, 1 for read- or write-only), 0 for
+// an unknown kind.
+int bandwidth_probe_streams(int kind);
+hipError_t launch_bandwidth_probe(int kind, const double *src, double *dst, size_t n, int cus, hipStream_t st);
+
+// flags[d] = 1 if the moments mom[4d .. 4d+3] (ncell[d] cells each) are all
+// finite, else 0.
+hipError_t launch_finite_flags(const double *const *mom, const uint32_t *ncell, int ndom, int32_t *flags,
+                               hipStream_t st);
 
 }  // namespace imx
 

@@ -222,6 +222,19 @@ def copy_bandwidth(nbytes, reps=20, device=0):
     return g.value, ms.value
 
 
+BANDWIDTH_PROBES = ("copy", "copy_nt", "copy_x4", "copy_x4_nt", "copy_gridstride", "copy_gridstride_nt",
+                    "read_x4", "read_x4_nt", "write_x4", "write_x4_nt")
+
+
+def bandwidth_probe(kind, nbytes, reps=20, device=0):
+    """GB/s (every byte read and written) and ms per launch of one of the
+    BANDWIDTH_PROBES stream kernels over nbytes bytes per stream."""
+    k = BANDWIDTH_PROBES.index(kind) if isinstance(kind, str) else int(kind)
+    g, ms = ctypes.c_double(), ctypes.c_float()
+    check(lib().imexlb_bandwidth_probe(int(device), k, int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(ms)))
+    return g.value, ms.value
+
+
 class Ensemble:
     """Many independent TGV / config-mode cavity domains stepped together by
     one persistent launch per step (optionally replayed from a HIP graph)."""
@@ -277,6 +290,17 @@ class Ensemble:
         shape = (dom.nz, dom.ny, dom.nx)
         out = [np.empty(shape) for _ in range(4)]
         self._chk(lib().imexlb_ensemble_get_moments(self.h, int(d), *[dptr(a) for a in out]))
+        return out
+
+    def status(self):
+        """Per-domain health after a step that wrote moments: a list of
+        booleans, True while the domain's moments are all finite."""
+        n = len(self.domains)
+        fin = (ctypes.c_int32 * n)()
+        bad = ctypes.c_int32()
+        self._chk(lib().imexlb_ensemble_status(self.h, fin, ctypes.byref(bad)))
+        out = [bool(v) for v in fin]
+        assert out.count(False) == bad.value
         return out
 
     def set_launch(self, block, cpt, grid):

@@ -160,9 +160,12 @@ int imexlb_set_f(imexlb_handle *h, const double *f);
  * rules read ghost values written in earlier steps; a phased handle of the
  * same case, mode and extents reads it.  The fused engine's faithful
  * cavity and sphere: "IMXLBCR1", as "IMXLBCK1" followed by the ghost ring
- * (27 populations over the six first-layer ghost faces, FP64); a sphere
- * handle must also have the same u0 and seed.  Each kind of
- * handle refuses the other kinds' files.  All resume at the saved step. */
+ * (27 populations over the six first-layer ghost faces, FP64).  Every
+ * reader must also have the writer's tau0 and rho0, and where the case uses
+ * them its lid speed (config cavity), inlet speed u0 (sphere), ball centre and
+ * radius (config sphere) and seed (faithful sphere); a file written with other
+ * physics is refused with IMEXLB_EINVAL.  Each kind of handle refuses the
+ * other kinds' files.  All resume at the saved step. */
 int imexlb_checkpoint_write(imexlb_handle *h, const char *path);
 int imexlb_checkpoint_read(imexlb_handle *h, const char *path);
 
@@ -181,6 +184,15 @@ const char *imexlb_strerror(int status);
 /* Device copy of `bytes` bytes (read + write), `reps` times: measured
  * bandwidth ceiling (GB/s counted as read + write bytes / time). */
 int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, float *ms_per_launch);
+/* Bandwidth ceiling probes (no counterpart in the reference), over `bytes`
+ * bytes per stream, `reps` timed launches after one warm-up:
+ *   0 copy, one 16-byte vector per lane    1 the same, non-temporal
+ *   2 copy, four 16-byte vectors per lane  3 the same, non-temporal
+ *   4 copy, grid-stride over 8 workgroups per CU   5 the same, non-temporal
+ *   6 read only, four vectors per lane     7 the same, non-temporal
+ *   8 write only, four vectors per lane    9 the same, non-temporal
+ * *gbps counts every byte read and written. */
+int imexlb_bandwidth_probe(int device, int kind, size_t bytes, int reps, double *gbps, float *ms_per_launch);
 
 /* Launch-shape autotuner (no counterpart in the reference).  Times every
  * workgroup size x cells per thread x XCD tile order x load variant on this
@@ -232,6 +244,14 @@ int imexlb_ensemble_initialize(imexlb_ensemble *e);
 int imexlb_ensemble_step(imexlb_ensemble *e, int nsteps, int mode, int moments_every);
 int imexlb_ensemble_step_timed(imexlb_ensemble *e, int nsteps, int mode, int moments_every, float *ms);
 int imexlb_ensemble_get_moments(imexlb_ensemble *e, int domain, double *p, double *u, double *v, double *w);
+/* Per-domain health after a step that wrote moments: finite[d] (n entries,
+ * may be NULL) is 1 while every moment of domain d is finite and 0 once the
+ * domain has blown up -- BGK has a stability limit at small tau0 (high Re on
+ * a coarse grid), and the seeded draw of imexlb_ensemble_generate reaches
+ * past it for some small domains.  *unstable (may be NULL) receives the
+ * number of zeros.  Other domains are unaffected: each steps independently.
+ * IMEXLB_ESTATE if the current step wrote no moments. */
+int imexlb_ensemble_status(imexlb_ensemble *e, int32_t *finite, int32_t *unstable);
 /* out[7]: cells, tiles, grid, block, cells per thread, device bytes, steps */
 int imexlb_ensemble_info(imexlb_ensemble *e, int64_t *out);
 int imexlb_ensemble_set_launch(imexlb_ensemble *e, int block, int cells_per_thread, int grid);

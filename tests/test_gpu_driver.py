@@ -4,11 +4,40 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "imexlb_amd", "bin", "imexlb")
+
+
+def oracle_diagnostics(tmp_path, case, mode, steps, keep_hardcoded=False):
+    """KE and mass of the oracle after `steps` steps of the input.in in
+    tmp_path, summed exactly (math.fsum) over the oracle's interior moments,
+    plus the sum of |3p| that scales the mass's rounding."""
+    import math
+
+    from imexlb_amd import System
+    from oracle.oracle import CONFIG, FAITHFUL, LDC, SPHERE, TGV, Oracle
+    cid = {"tgv": TGV, "cavity": LDC, "sphere": SPHERE}[case]
+    s = System(str(tmp_path / "input.in"), case, keep_hardcoded=keep_hardcoded)
+    o = Oracle(cid, s.sx, s.sy, s.sz, tau0=s.tau, rho0=s.rho0, u0=s.u0,
+               mode=FAITHFUL if mode == "faithful" else CONFIG)
+    o.Initialize()
+    o.run(steps)
+    p, u, v, w = (a.ravel() for a in o.moments())
+    ke = 0.5 * math.fsum(u * u + v * v + w * w)
+    mass = math.fsum(3.0 * p)
+    return ke, mass, math.fsum(np.abs(3.0 * p))
+
+
+def assert_diag(got, want):
+    """The library's two-pass FP64 reduction against the exactly rounded sum
+    of the same fields: within 1e-14 of the sum's scale."""
+    (ke, mass), (ke_o, mass_o, mass_scale) = got, want
+    assert abs(ke - ke_o) <= 1e-14 * ke_o, (ke, ke_o)
+    assert abs(mass - mass_o) <= 1e-14 * mass_scale, (mass, mass_o)
 
 
 def run(tmp_path, text, args):
@@ -35,21 +64,35 @@ def test_driver_tgv_matches_python(tmp_path, storage):
     assert abs(ke - ke2) <= 1e-14 * ke2 and abs(mass - mass2) <= 1e-14 * mass2
 
 
-def test_driver_cavity_faithful_and_config(tmp_path):
+@pytest.mark.parametrize("mode,engine", [("faithful", "fused"), ("faithful", "phased"), ("config", "fused"),
+                                         ("config", "phased")])
+def test_driver_cavity_matches_oracle(tmp_path, mode, engine):
     text = "1.0  0.1  10\n0.1  20  10\n24 24 24\n"
-    ke_f, _, out = run(tmp_path, text, ["--case", "cavity", "--mode", "faithful"])
-    ke_c, _, _ = run(tmp_path, text, ["--case", "cavity", "--mode", "config"])
-    assert ke_f > 0 and ke_c > 0
+    ke, mass, out = run(tmp_path, text, ["--case", "cavity", "--mode", mode, "--engine", engine])
+    assert_diag((ke, mass), oracle_diagnostics(tmp_path, "cavity", mode, 20))
 
 
-def test_driver_sphere_and_fast_path(tmp_path):
+@pytest.mark.parametrize("engine", ["fused", "phased"])
+def test_driver_sphere_matches_oracle(tmp_path, engine):
     text = "1.0  8.0  100\n0.1  100  10\n72 20 20\n"
-    ke, mass, _ = run(tmp_path, text, ["--case", "sphere", "--steps", "20"])
-    assert ke > 0
+    ke, mass, _ = run(tmp_path, text, ["--case", "sphere", "--steps", "20", "--engine", engine])
+    assert_diag((ke, mass), oracle_diagnostics(tmp_path, "sphere", "faithful", 20))
+
+
+def test_driver_fast_path(tmp_path):
+    """--fast (imexlb_step over the whole run) ends where the six-call loop
+    ends, also with inter = 0, where only the last step writes moments."""
     text = "1.0  0.1  100\n0.05  40  10\n32 32 32\n"
-    ke1, _, _ = run(tmp_path, text, ["--case", "tgv", "--honour-input"])
-    ke2, _, out = run(tmp_path, text, ["--case", "tgv", "--honour-input", "--fast"])
-    assert ke1 == ke2
+    ke1, m1, _ = run(tmp_path, text, ["--case", "tgv", "--honour-input"])
+    ke2, m2, out = run(tmp_path, text, ["--case", "tgv", "--honour-input", "--fast"])
+    assert (ke1, m1) == (ke2, m2)
+    assert_diag((ke1, m1), oracle_diagnostics(tmp_path, "tgv", "faithful", 40))
+    text = "1.0  0.1  100\n0.05  13  0\n24 20 16\n"
+    ke3, m3, _ = run(tmp_path, text, ["--case", "tgv", "--honour-input", "--fast", "--storage", "aa"])
+    assert_diag((ke3, m3), oracle_diagnostics(tmp_path, "tgv", "faithful", 13))
+    text = "1.0  0.1  10\n0.1  13  0\n20 16 12\n"
+    ke4, m4, _ = run(tmp_path, text, ["--case", "cavity", "--mode", "faithful", "--honour-input", "--fast"])
+    assert_diag((ke4, m4), oracle_diagnostics(tmp_path, "cavity", "faithful", 13))
 
 
 @pytest.mark.parametrize("storage", ["pull", "aa"])
@@ -66,11 +109,13 @@ def test_driver_restart_continues_the_run(tmp_path, storage):
     assert (ke, mass) == (ke_full, mass_full)
 
 
-def test_driver_restart_faithful_cavity(tmp_path):
-    """The reference's own cavity (faithful mode, phased engine): 12 steps, a
-    restart file with the ghost cells, then 9 more end where 21 steps end."""
+@pytest.mark.parametrize("engine", ["fused", "phased"])
+def test_driver_restart_faithful_cavity(tmp_path, engine):
+    """The reference's own cavity (faithful mode): 12 steps, a restart file
+    with the ghost cells (the fused engine's ghost ring, or the phased
+    engine's ghost-3 state), then 9 more end where 21 steps end."""
     text = "1.0  0.1  10\n0.1  21  0\n20 16 12\n"
-    common = ["--case", "cavity", "--mode", "faithful", "--honour-input"]
+    common = ["--case", "cavity", "--mode", "faithful", "--honour-input", "--engine", engine]
     ke_full, mass_full, _ = run(tmp_path, text, common)
     ck = str(tmp_path / "cav.ck")
     run(tmp_path, text, common + ["--steps", "12", "--checkpoint", ck])
@@ -80,15 +125,15 @@ def test_driver_restart_faithful_cavity(tmp_path):
 
 
 def test_driver_faithful_cavity_fused_engine(tmp_path):
-    """--engine fused runs the reference's own cavity rules on the fused
-    engine's ghost ring, to the final KE and mass of the phased engine.  The
+    """The default fused engine runs the reference's own cavity rules on its
+    ghost ring, to the final KE and mass of the phased engine.  The
     fields are the same bits (tests/test_gpu_parity.py); the two engines'
     FP64 reductions add them in different orders, so the sums agree to
     rounding."""
     text = "1.0  0.1  10\n0.1  15  0\n20 16 12\n"
     common = ["--case", "cavity", "--mode", "faithful", "--honour-input"]
-    ke_p, mass_p, _ = run(tmp_path, text, common)
-    ke_f, mass_f, _ = run(tmp_path, text, common + ["--engine", "fused"])
+    ke_p, mass_p, _ = run(tmp_path, text, common + ["--engine", "phased"])
+    ke_f, mass_f, _ = run(tmp_path, text, common)
     assert abs(ke_f - ke_p) <= 1e-14 * ke_p and abs(mass_f - mass_p) <= 1e-14 * abs(mass_p)
 
 
@@ -116,17 +161,17 @@ def test_driver_output_files(tmp_path):
         assert np.array_equal(data[3 + k], a)
 
 
-@pytest.mark.parametrize("case,extra", [("tgv", []), ("cavity", ["--mode", "faithful"]),
-                                        ("sphere", ["--mode", "faithful"])])
-def test_driver_runs_the_reference_inputs(tmp_path, case, extra):
-    """The reference's own input.in files, unchanged (tgv 256^3 with its
-    hard-coded 100 steps, cavity 100^3, sphere 512x128x128), through the
-    driver's six-call loop: the run completes with the reference's report
-    lines and finite diagnostics."""
-    import math
+@pytest.mark.parametrize("case,steps", [("tgv", 4), ("cavity", 40), ("sphere", 10)])
+def test_driver_runs_the_reference_inputs(tmp_path, case, steps):
+    """The reference's own input.in files, unchanged (tgv 256^3 Re 1600,
+    cavity 100^3, sphere 512x128x128), through the driver's six-call loop on
+    the default (fused) engine, with the run shortened to `steps` steps: the
+    reference's report lines, and the final KE and mass of the oracle's
+    fields after the same steps."""
     text = open(os.path.join(ROOT, "tests", "golden", f"input_{case}.in")).read()
-    args = ["--case", case] + extra + (["--steps", "40"] if case != "tgv" else [])
+    args = ["--case", case, "--steps", str(steps)]
+    print(f"[driver] reference input {case}, {steps} steps", flush=True)
     ke, mass, out = run(tmp_path, text, args)
-    assert math.isfinite(ke) and math.isfinite(mass) and ke > 0
     for line in ("Avg time spent in Pack:", "Avg time spent in Stream:", "Avg. Total Solver Time:", "Throughput:"):
         assert line in out
+    assert_diag((ke, mass), oracle_diagnostics(tmp_path, case, "faithful", steps, keep_hardcoded=True))

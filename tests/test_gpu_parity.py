@@ -271,6 +271,34 @@ def test_restart_files_name_their_engine(tmp_path):
         LBM(n[0] + 1, n[1], n[2], tau, u0=0.05, engine="phased").restart(tmp_path / "ph.ck")
 
 
+def test_restart_refuses_other_physics(tmp_path):
+    """A restart file written with another BGK rate, density, lid speed, inlet
+    speed or ball is refused (IMEXLB_EINVAL) instead of continuing other
+    physics; the same parameters read it."""
+    from imexlb_amd import LBM
+    from imexlb_amd._lib import ImexlbError
+    n = (12, 10, 8)
+    base = dict(case="cavity", mode="config", lid_u=0.08, rho0=1.0)
+    a = LBM(*n, 0.1, **base)
+    a.Initialize()
+    a.step(3)
+    a.checkpoint(tmp_path / "cav.ck")
+    LBM(*n, 0.1, **base).restart(tmp_path / "cav.ck")
+    for tau, kw in [(0.11, {}), (0.1, dict(rho0=1.01)), (0.1, dict(lid_u=0.09))]:
+        with pytest.raises(ImexlbError) as err:
+            LBM(*n, tau, **{**base, **kw}).restart(tmp_path / "cav.ck")
+        assert err.value.status == -1, err.value
+    sph = dict(case="sphere", mode="config", u0=0.05, sphere=(6.0, 5.0, 4.0, 2.5))
+    b = LBM(20, 10, 8, 0.05, **sph)
+    b.Initialize()
+    b.step(2)
+    b.checkpoint(tmp_path / "sph.ck")
+    LBM(20, 10, 8, 0.05, **sph).restart(tmp_path / "sph.ck")
+    for kw in (dict(u0=0.04), dict(sphere=(6.0, 5.0, 4.0, 2.0)), dict(sphere=(7.0, 5.0, 4.0, 2.5))):
+        with pytest.raises(ImexlbError):
+            LBM(20, 10, 8, 0.05, **{**sph, **kw}).restart(tmp_path / "sph.ck")
+
+
 INIT_CASES = [(TGV, FAITHFUL, "fused", "pull", "f64"), (TGV, FAITHFUL, "fused", "aa", "f64"),
               (TGV, FAITHFUL, "fused", "aa", "f16"), (TGV, FAITHFUL, "fused", "pull", "f32"),
               (LDC, CONFIG, "fused", "aa", "f64"), (SPHERE, CONFIG, "fused", "pull", "f64"),
