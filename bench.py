@@ -71,6 +71,26 @@ def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None, pl
     return g, ms, info
 
 
+def placement_summary(placement, bytes_lup):
+    """First-allocation, median and kept rates of a placement search, as
+    GLUPS and as the fraction of the 8 TB/s peak."""
+    if not placement:
+        return {}
+    rates = [r for r in placement["glups_each"] if r]
+    frac = lambda g: g * bytes_lup / PEAK_GBS
+    first, med, kept = rates[0], float(np.median(rates)), placement["glups_each"][placement["kept"]]
+    return {"glups_first_alloc": first, "glups_median": med, "glups_kept": kept,
+            "frac_first_alloc": frac(first), "frac_median": frac(med), "frac_kept": frac(kept)}
+
+
+def trials_for_budget(n, dtype, storage, default, budget_gb):
+    """Placement candidates that fit in `budget_gb` of device memory held at
+    once (every candidate keeps its own lattice until the search ends)."""
+    elem = {"f64": 8, "f32": 4, "f16": 2}[dtype]
+    lat_gb = 27 * n ** 3 * elem * (2 if storage == "pull" else 1) / 1e9
+    return max(1, min(default, int(budget_gb // lat_gb))), lat_gb
+
+
 def tgv_analytic_velocity(n, u0, re, steps):
     """The Taylor-Green velocity of the initial field (tgv/src/lbm.cpp:197-200,
     pi = 3.1415926) decayed viscously: |k|^2 = 3 (2 pi / n)^2, u(t) = u(0)
@@ -112,27 +132,77 @@ def traffic_from_profiles(workload):
         return None
 
 
+def host_cpu():
+    """CPU model, logical CPUs this process may run on, and host RAM."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    ram_gb = None
+    try:
+        with open("/proc/meminfo") as f:
+            ram_gb = round(int(f.readline().split()[1]) / 2 ** 20, 1)
+    except (OSError, ValueError, IndexError):
+        pass
+    return model, avail, ram_gb
+
+
 def cpu_baseline(budget_s):
-    """The oracle (reference-structured CPU restatement, OpenMP) on a bounded
-    sample of the same workload: TGV 128^3, same U0 and Re, FP64."""
-    from oracle.oracle import TGV, Oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    """The oracle -- the reference's phases (Collision, the 26-message ghost-3
+    periodic self-copy, the two streaming passes, Update) on its ghost-3
+    f(q, x, y, z) layout in FP64 with OpenMP -- on a bounded sample of the
+    same workload: TGV 128^3, same U0 and Re.  It is compiled for this host
+    (-march=native) into a temporary directory, runs on every CPU this process
+    may use (OMP_PROC_BIND=spread, OMP_PLACES=cores), and is timed phase by
+    phase after one warm-up step, as the reference's driver times them."""
+    import tempfile
+    model, avail, ram_gb = host_cpu()
+    threads = min(avail, int(os.environ.get("OMP_NUM_THREADS", "0")) or avail)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from oracle import oracle as orc
+    build = "generic x86-64-v3 build (native build failed)"
+    try:
+        path = orc.build_native(tempfile.mkdtemp(prefix="imexlb_oracle_native_"))
+        orc.use_library(path)
+        build = "-march=native build"
+    except Exception as exc:  # the generic in-tree build still times the same code
+        log(f"[bench] CPU baseline: native oracle build failed ({exc}); timing the generic build")
     n = 128
-    o = Oracle(TGV, n, n, n, tau0=tgv_tau(n, 1600, 0.05), u0=0.05, threads=threads)
+    o = orc.Oracle(orc.TGV, n, n, n, tau0=tgv_tau(n, 1600, 0.05), u0=0.05, threads=threads)
     o.Initialize()
     o.run(1)  # warm-up
+    phases = ("Collision", "pack", "exchange", "unpack", "Streaming", "Update")
+    spent = dict.fromkeys(phases, 0.0)
     t0 = time.perf_counter()
     k = 0
     while True:
-        o.run(1)
+        for name in phases:
+            ts = time.perf_counter()
+            getattr(o, name)()
+            spent[name] += time.perf_counter() - ts
         k += 1
         el = time.perf_counter() - t0
         if el > budget_s or k >= 200:
             break
     glups = n ** 3 * k / el / 1e9
     return {"value": glups, "unit": "GLUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle (reference phases, ghost 3, FP64, OpenMP) TGV {n}^3 Re 1600, "
-                      f"{k} steps in {el:.1f} s after 1 warm-up step"}
+            "sample": f"oracle (reference phases, ghost 3, FP64, OpenMP, {build}) TGV {n}^3 Re 1600, "
+                      f"{k} steps in {el:.1f} s after 1 warm-up step",
+            "cpu_model": model, "nproc": avail, "host_ram_gb": ram_gb,
+            "omp": {"threads": threads, "proc_bind": os.environ["OMP_PROC_BIND"],
+                    "places": os.environ["OMP_PLACES"]},
+            "gb_per_s": glups * 432.0,
+            "ms_per_step_by_phase": {name: 1e3 * t / k for name, t in spent.items()}}
 
 
 def main():
@@ -152,7 +222,10 @@ def main():
     ap.add_argument("--no-lowp", action="store_true")
     ap.add_argument("--placements", type=int, default=0,
                     help="lattice allocations timed before the run, fastest kept (1: first only; "
-                         "0: 8 / 8 / 12 for FP64 / FP32 / FP16, about 235 / 120 / 90 GB at 512^3)")
+                         "0: up to 8 / 8 / 12 for FP64 / FP32 / FP16, as many as --placement-budget-gb holds)")
+    ap.add_argument("--placement-budget-gb", type=float, default=128.0,
+                    help="device memory the placement search may hold at once (all candidates live "
+                         "until it ends): 4 FP64, 8 FP32 and 12 FP16 lattices at 512^3 AA")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -161,14 +234,21 @@ def main():
         sys.exit(2)
 
     import imexlb_amd  # noqa: F401  (fails loudly without libimexlb.so)
-    from imexlb_amd import copy_bandwidth
+    from imexlb_amd import BANDWIDTH_PROBES, bandwidth_probe, copy_bandwidth
 
     n, K, W = a.n, a.steps, a.warmup
     cells = n ** 3
     log(f"[bench] TGV {n}^3 {a.storage} FP64: warmup {W}, timed {K}")
     launch = tuple(int(v) for v in a.launch.split(",")) if a.launch else None
-    trials = {dt: a.placements or {"f64": 8, "f32": 8, "f16": 12}[dt] for dt in ("f64", "f32", "f16")}
+    trials, held_gb = {}, {}
+    for dt in ("f64", "f32", "f16"):
+        t, lat_gb = trials_for_budget(n, dt, a.storage, a.placements or {"f64": 8, "f32": 8, "f16": 12}[dt],
+                                      a.placement_budget_gb)
+        trials[dt] = t
+        held_gb[dt] = round(t * lat_gb, 1)
     g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune, launch, trials["f64"])
+    if info64["placement"]:
+        info64["placement"]["held_gb"] = held_gb["f64"]
     glups64 = cells * K / (ms64 * 1e-3) / 1e9
     per_launch_ms = ms64 / K
     bytes_lup = info64["bytes_per_lup"]
@@ -185,6 +265,9 @@ def main():
             log(f"[bench] TGV {n}^3 {a.storage} {dt}")
             g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re, not a.no_tune, None, trials[dt])
             gl = cells * K / (ms * 1e-3) / 1e9
+            if info["placement"]:
+                info["placement"]["held_gb"] = held_gb[dt]
+                info["placement"].update(placement_summary(info["placement"], info["bytes_per_lup"]))
             m = g.moments()
             err = vel_l2(m, m64)
             an = vel_l2_analytic(m, n, a.u0, a.re, W + K + 1)
@@ -197,8 +280,17 @@ def main():
             del g, m
     del m64
 
-    gbps_copy, _ = copy_bandwidth(4 << 30, reps=10)
-    log(f"[bench] device copy {gbps_copy:.0f} GB/s")
+    # bandwidth ceiling: the plain copy and the tuned stream probes (copies,
+    # read-only, write-only) over 4 GiB per stream; the best copy is the
+    # measured ceiling for a kernel with one read and one write per value
+    gbps_plain, _ = copy_bandwidth(4 << 30, reps=10)
+    probes = {"copy_plain": gbps_plain}
+    for kind in BANDWIDTH_PROBES:
+        probes[kind] = bandwidth_probe(kind, 4 << 30, reps=10)[0]
+    best_copy = max((k for k in probes if k.startswith("copy")), key=probes.get)
+    gbps_copy = probes[best_copy]
+    log(f"[bench] device copy {gbps_plain:.0f} GB/s plain, {gbps_copy:.0f} GB/s best ({best_copy}); probes "
+        + ", ".join(f"{k} {v:.0f}" for k, v in probes.items()))
     workload = f"tgv{n}_{a.storage}_f64"
     traffic = traffic_from_profiles(workload)
     cpu = None
@@ -230,7 +322,15 @@ def main():
                      "bytes_per_lup": bytes_lup, "launch": {k: info64[k] for k in ("block", "cells_per_thread", "xcd_remap", "nt_loads")},
                      "kernel": "k_fused (FP64, periodic, "
                      + ("AA" if a.storage == "aa" else "pull") + ")",
-                     "measured_copy_GBps": gbps_copy, "placement": info64["placement"]},
+                     "measured_copy_GBps": gbps_copy, "measured_copy_kernel": best_copy,
+                     "bandwidth_probes_GBps": {k: round(v, 1) for k, v in probes.items()},
+                     "placement": info64["placement"],
+                     **placement_summary(info64["placement"], bytes_lup),
+                     "value_basis": ("value and frac: the timed run on the fastest of "
+                                     f"{trials['f64']} lattice allocations ({held_gb['f64']} GB held during the "
+                                     "search); glups/frac_first_alloc is the first allocation, "
+                                     "glups/frac_median the median candidate" if trials["f64"] > 1 else
+                                     "value and frac: the timed run on the first lattice allocation")},
         "cpu_baseline": cpu,
         "tgv_error": {"after_steps": W + K + 1, "f64_vel_l2_vs_analytic": an64,
                       "note": "analytic = initial TGV field decayed by exp(-3 nu k^2 t), the linear solution; "

@@ -387,21 +387,32 @@ typedef struct {
     int i0, i1, j0, j1, k0, k1;
 } box;
 
+/* Offset of (ii, i, j, k) in a box's staging buffer: the subview's index
+ * order, so every (ii, i) slab is a contiguous run and the slabs can be
+ * copied by separate threads (Kokkos-OpenMP runs deep_copy in parallel). */
+static size_t box_off(box b, int ii, int i)
+{
+    return ((size_t)ii * (size_t)(b.i1 - b.i0) + (size_t)(i - b.i0)) * (size_t)(b.j1 - b.j0) * (size_t)(b.k1 - b.k0);
+}
 static void box_to_buf(const orc_t *s, box b, double *buf)
 {
-    size_t n = 0;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int ii = 0; ii < Q; ii++)
-        for (int i = b.i0; i < b.i1; i++)
+        for (int i = b.i0; i < b.i1; i++) {
+            size_t n = box_off(b, ii, i);
             for (int j = b.j0; j < b.j1; j++)
                 for (int k = b.k0; k < b.k1; k++) buf[n++] = s->f[F(s, ii, i, j, k)];
+        }
 }
 static void buf_to_box(orc_t *s, box b, const double *buf)
 {
-    size_t n = 0;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int ii = 0; ii < Q; ii++)
-        for (int i = b.i0; i < b.i1; i++)
+        for (int i = b.i0; i < b.i1; i++) {
+            size_t n = box_off(b, ii, i);
             for (int j = b.j0; j < b.j1; j++)
                 for (int k = b.k0; k < b.k1; k++) s->f[F(s, ii, i, j, k)] = buf[n++];
+        }
 }
 
 /* Pack boxes and unpack boxes of the periodic TGV halo, indexed by message tag

@@ -47,6 +47,23 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def build_native(out_dir):
+    """Compile the same source for this host (-march=native) into out_dir:
+    the timing build of the CPU baseline.  Returns the library's path."""
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={out_dir}"], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    return os.path.join(out_dir, "libimexlb_oracle.so")
+
+
+def use_library(path):
+    """Load the oracle from `path` instead of oracle/build/ (before the first
+    use in this process)."""
+    global LIB_PATH
+    if _lib is not None and path != LIB_PATH:
+        raise RuntimeError("oracle: a library is already loaded")
+    LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:
