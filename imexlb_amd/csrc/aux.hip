@@ -19,28 +19,30 @@ __device__ __forceinline__ double feq_f64(int i, double p, double u, double v, d
     return WVAL[WCLASS[i]] * p * 3.0 + WVAL[WCLASS[i]] * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
 }
 
+// Population f of direction i (weight class of slot `slot`, which is i or
+// bb(i)) in the stored convention, g = f - s with s the direction's shift.
 template <typename S>
-__device__ __forceinline__ void put(S *lat, uint64_t ps, int i, uint32_t c, double f, double shift)
+__device__ __forceinline__ void put(S *lat, uint64_t ps, int slot, uint32_t c, double f, double s)
 {
-    S *p = lat + (uint64_t)i * ps + c;
+    S *p = lat + (uint64_t)slot * ps + c;
     if constexpr (std::is_same<S, double>::value)
         *p = f;
     else if constexpr (std::is_same<S, float>::value)
-        *p = (float)(f - WVAL[WCLASS[i]] * shift);
+        *p = (float)(f - s);
     else
-        *p = (_Float16)((float)(f - WVAL[WCLASS[i]] * shift) * H_SCALE[WCLASS[i]]);
+        *p = (_Float16)((float)(f - s) * H_SCALE[WCLASS[slot]]);
 }
 
 template <typename S>
-__device__ __forceinline__ double get(const S *lat, uint64_t ps, int i, uint32_t c, double shift)
+__device__ __forceinline__ double get(const S *lat, uint64_t ps, int slot, uint32_t c, double s)
 {
-    const S *p = lat + (uint64_t)i * ps + c;
+    const S *p = lat + (uint64_t)slot * ps + c;
     if constexpr (std::is_same<S, double>::value)
         return *p;
     else if constexpr (std::is_same<S, float>::value)
-        return (double)*p + WVAL[WCLASS[i]] * shift;
+        return (double)*p + s;
     else
-        return (double)((float)*p * H_INV_SCALE[WCLASS[i]]) + WVAL[WCLASS[i]] * shift;
+        return (double)((float)*p * H_INV_SCALE[WCLASS[slot]]) + s;
 }
 
 // Initial state = f0 = feq(p, u) of the analytic fields.  The reference's
@@ -93,7 +95,7 @@ __global__ void k_init(const InitParams P)
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const double f = feq_f64(i, p, u, v, w);
-        put<S>(lat, P.pstride, P.layout == 1 ? BB[i] : i, c, f, P.shift);
+        put<S>(lat, P.pstride, P.layout == 1 ? BB[i] : i, c, f, P.sh[i]);
     }
     // the fields themselves are the moments of step 0, as the reference's
     // Initialize leaves them in p, ua, va, wa; a solid cell reports a fluid at
@@ -165,8 +167,8 @@ __global__ void k_get_f(const XferParams P)
         slot_of(P, i, c, slot, cell);
         double v = 0.0;
         // slot is i or bb(i); both share the weight class of i
-        if (slot == i) v = get<S>(lat, P.pstride, i, cell, P.shift);
-        else v = get<S>(lat, P.pstride, BB[i], cell, P.shift);
+        if (slot == i) v = get<S>(lat, P.pstride, i, cell, P.sh[i]);
+        else v = get<S>(lat, P.pstride, BB[i], cell, P.sh[i]);
         P.buf[(uint64_t)i * P.ncell + c] = v;
     }
 }
@@ -183,8 +185,8 @@ __global__ void k_set_f(const XferParams P)
         uint32_t cell;
         slot_of(P, i, c, slot, cell);
         const double v = P.buf[(uint64_t)i * P.ncell + c];
-        if (slot == i) put<S>(lat, P.pstride, i, cell, v, P.shift);
-        else put<S>(lat, P.pstride, BB[i], cell, v, P.shift);
+        if (slot == i) put<S>(lat, P.pstride, i, cell, v, P.sh[i]);
+        else put<S>(lat, P.pstride, BB[i], cell, v, P.sh[i]);
     }
 }
 
@@ -299,7 +301,7 @@ __global__ void __launch_bounds__(256) k_force_pass1(const FusedParams P, double
             zd = zd < 0 ? zd + P.nz : (zd >= P.nz ? zd - P.nz : zd);
             const uint32_t cd = ((uint32_t)zd * P.ny + (uint32_t)yd) * P.nx + (uint32_t)xd;
             if (P.mask[cd] != 1) continue;
-            const double f = 2.0 * get<S>(lat, P.pstride, i, c, P.shift);
+            const double f = 2.0 * get<S>(lat, P.pstride, i, c, P.sh[i]);
             acc[0] += f * (double)EX[i];
             acc[1] += f * (double)EY[i];
             acc[2] += f * (double)EZ[i];

@@ -34,7 +34,9 @@ struct imexlb_handle {
     uint64_t pstride = 0;
     uint32_t ncell = 0;
     size_t elem = 8;
-    double shift = 0.0;
+    double shift = 0.0;      // S: FP32/FP16 lattices store f - s_i with sum_i s_i = S
+    double ush = 0.0;        // the shift's free-stream x velocity (reduced-precision channel)
+    double sh[Q] = {};       // s_i (storage_shift)
     double corr[Q] = {};
     double feq_in[Q] = {};
     double *tgv_tab = nullptr;
@@ -177,6 +179,8 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.tau0 = c.tau0;
     P.omega_f = (float)(1.0 / (c.tau0 + 0.5));
     P.shift = h->shift;
+    P.ush = h->ush;
+    std::memcpy(P.sh, h->sh, sizeof P.sh);
     std::memcpy(P.corr, h->corr, sizeof P.corr);
     std::memcpy(P.feq_in, h->feq_in, sizeof P.feq_in);
     const uint32_t tile = (uint32_t)h->block * (uint32_t)h->cpt;
@@ -356,8 +360,13 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
     h->cpt = c.cells_per_thread ? c.cells_per_thread : 1;
     h->xcd = c.xcd_remap < 0 ? 1 : c.xcd_remap;
     h->ntl = c.nt_loads < 0 ? 1 : (c.nt_loads ? 1 : 0);
-    // stored shift S: f = g + w S for reduced precision; FP64 stores f itself
-    if (c.dtype != IMEXLB_F64) h->shift = c.rho0;
+    // reduced precision stores g_i = f_i - s_i (storage_shift); FP64 stores f itself
+    if (c.dtype != IMEXLB_F64) {
+        h->shift = c.rho0;
+        // the open channel stores its departure from the inflow equilibrium
+        if (c.case_id == IMEXLB_CASE_SPHERE && c.mode == IMEXLB_MODE_CONFIG) h->ush = c.u0;
+        storage_shift(h->shift, h->ush, h->sh);
+    }
     const size_t lat_bytes = (size_t)Q * h->pstride * h->elem;
     const int nlat = c.storage == IMEXLB_PULL ? 2 : 1;
     for (int k = 0; k < nlat; ++k) {
@@ -408,7 +417,10 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
             const double udu = u * u + v * v + ww * ww;
             const double eu2 = edu * edu;
             const double feq = w(i) * pin * 3.0 + w(i) * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
-            h->feq_in[i] = c.dtype == IMEXLB_F64 ? feq : feq - w(i) * h->shift;
+            h->feq_in[i] = c.dtype == IMEXLB_F64 ? feq : feq - h->sh[i];
+            // a link bounced off the ball arrives in the other direction's
+            // stored convention
+            h->corr[i] = c.dtype == IMEXLB_F64 ? 0.0 : h->sh[BB[i]] - h->sh[i];
         }
         // mask: 1 solid, 2 fluid with a solid neighbour (y, z periodic)
         std::vector<uint8_t> m((size_t)h->ncell, 0);
@@ -518,7 +530,7 @@ int imexlb_initialize(imexlb_handle *h)
     P.cs2 = 1.0 / 3.0;
     // the reference's cavity starts from p = 0, so f = 0 (lid_driven_cavity/src/lbm.cpp:196)
     P.p0 = bc_of(c) == BC_FAITHFUL ? 0.0 : c.rho0 * P.cs2;
-    P.shift = h->shift;
+    std::memcpy(P.sh, h->sh, sizeof P.sh);
     P.mask = h->mask;
     for (int k = 0; k < 4; ++k) P.mom[k] = h->mom[k];
     P.divx.init(c.nx);
@@ -698,7 +710,7 @@ static int xfer(imexlb_handle *h, double *host, bool get)
     P.nz = c.nz;
     P.ncell = h->ncell;
     P.pstride = h->pstride;
-    P.shift = h->shift;
+    std::memcpy(P.sh, h->sh, sizeof P.sh);
     P.bc = bc_of(c);
     std::memcpy(P.corr, h->corr, sizeof P.corr);
     if (c.storage == IMEXLB_PULL) {

@@ -40,16 +40,19 @@ __device__ __forceinline__ void tile_run(const FusedParams &P, uint32_t first, b
     using R = typename RealOf<S>::type;
     CellCtx k[CPT];
     R f[CPT][Q];
+    const S *at[CPT][Q];
+    // every member's populations span at most 4 GiB (imexlb_ensemble_create
+    // checks): 32-bit offsets inside a population
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         cell_setup<BC>(P, first + (uint32_t)r * blockDim.x + threadIdx.x, k[r]);
-        if (k[r].live) cell_gather<S, BC, PAT, NTL>(P, k[r], f[r]);
+        if (k[r].live) cell_gather<S, BC, PAT, NTL, true>(P, k[r], f[r], at[r]);
     }
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         if (!k[r].live) continue;
         cell_collide<S, BC>(P, k[r], f[r], wm);
-        cell_scatter<S, BC, PAT>(P, k[r], f[r]);
+        cell_scatter<S, BC, PAT, true, PAT == PAT_AA_ODD && CPT == 1>(P, k[r], f[r], at[r]);
     }
 }
 
@@ -363,6 +366,8 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
     for (int d = 0; d < n; ++d) {
         const uint64_t nc = (uint64_t)doms[d].nx * doms[d].ny * doms[d].nz, per = 256 / elem;
         pstride[d] = (nc + per - 1) / per * per + (uint64_t)IMEXLB_DEFAULT_PAD / elem;
+        // the member kernels address a population with 32-bit byte offsets
+        if (pstride[d] * elem > (1ull << 32)) return bail(IMEXLB_EINVAL);
         for (int k = 0; k < nl; ++k) {
             e->lat_off[2 * d + k] = e->pool_bytes;
             e->pool_bytes += ((size_t)Q * pstride[d] * elem + 4095) / 4096 * 4096;
@@ -403,6 +408,7 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
             P.tau0 = D.tau0;
             P.omega_f = (float)(1.0 / (D.tau0 + 0.5));
             P.shift = dtype == IMEXLB_F64 ? 0.0 : D.rho0;
+            if (dtype != IMEXLB_F64) storage_shift(P.shift, 0.0, P.sh);
             P.bc = D.case_id == IMEXLB_CASE_TGV ? BC_PERIODIC : BC_CAVITY;
             if (D.case_id == IMEXLB_CASE_CAVITY)
                 for (int i = 0; i < Q; ++i) P.corr[i] = 6.0 * w(i) * D.rho0 * ((double)EX[i] * D.lid_u);
@@ -455,7 +461,7 @@ int imexlb_ensemble_initialize(imexlb_ensemble *e)
         P.rho0 = D.rho0;
         P.cs2 = 1.0 / 3.0;
         P.p0 = D.rho0 * P.cs2;
-        P.shift = F.shift;
+        std::memcpy(P.sh, F.sh, sizeof P.sh);
         for (int k = 0; k < 4; ++k) P.mom[k] = e->mom[4 * d + k];
         P.divx.init(D.nx);
         P.divy.init(D.ny);

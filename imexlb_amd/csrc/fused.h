@@ -36,12 +36,12 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *glb(T *p)
 template <typename S> struct RealOf { using type = float; };
 template <> struct RealOf<double> { using type = double; };
 
-// Load the stored value at element `at` of the lattice, as a population of
-// weight class `cls` (FP16 storage is scaled per class).
+// Load the stored value at p, as a population of weight class `cls` (FP16
+// storage is scaled per class).
 template <typename S, bool NT = false>
-__device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t at, int cls)
+__device__ __forceinline__ typename RealOf<S>::type ld_ptr(const S *ptr, int cls)
 {
-    const auto *p = glb(lat + at);
+    const auto *p = glb(ptr);
     S x;
     if constexpr (NT) x = __builtin_nontemporal_load(p);
     else x = *p;
@@ -51,10 +51,35 @@ __device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t
         return x;
 }
 
+// The value at element `at` of the lattice.
 template <typename S, bool NT = false>
+__device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t at, int cls)
+{
+    return ld_ptr<S, NT>(lat + at, cls);
+}
+
+// Cell c of population i.  W32: through population i's array as a uniform
+// base plus the cell's byte offset as a 32-bit value -- base + zext(offset)
+// is the global saddr address form, one VGPR per address instead of a 64-bit
+// pair, which keeps the AA odd step (27 neighbour addresses loaded and
+// stored) at the register count of the other steps.  Valid while one
+// population spans at most 4 GiB, which the launch checks.
+template <typename S, bool W32>
+__device__ __forceinline__ S *pop_ptr(S *lat, uint64_t pstride, int i, uint32_t c)
+{
+    if constexpr (W32) {
+        auto *base = (typename std::conditional<std::is_const<S>::value, const char, char>::type *)(
+            lat + (uint64_t)i * pstride);
+        return (S *)(base + (uint32_t)(c * (uint32_t)sizeof(S)));
+    } else {
+        return lat + (uint64_t)i * pstride + c;
+    }
+}
+
+template <typename S, bool NT = false, bool W32 = false>
 __device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_t pstride, int i, uint32_t c)
 {
-    return ld_at<S, NT>(lat, (uint64_t)i * pstride + c, WCLASS[i]);
+    return ld_ptr<S, NT>(pop_ptr<const S, W32>(lat, pstride, i, c), WCLASS[i]);
 }
 
 // NTL (non-temporal loads) variant of the gather: lines that no other
@@ -71,14 +96,20 @@ __device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_
 #define IMX_STORE_POLICY 2
 #endif
 template <typename S, bool NT = true>
-__device__ __forceinline__ void st_pop(S *lat, uint64_t pstride, int i, uint32_t c, typename RealOf<S>::type v)
+__device__ __forceinline__ void st_ptr(S *ptr, int cls, typename RealOf<S>::type v)
 {
-    auto *p = glb(lat + (uint64_t)i * pstride + c);
+    auto *p = glb(ptr);
     S x;
-    if constexpr (std::is_same<S, _Float16>::value) x = (_Float16)(v * H_SCALE[WCLASS[i]]);
+    if constexpr (std::is_same<S, _Float16>::value) x = (_Float16)(v * H_SCALE[cls]);
     else x = v;
     if constexpr (NT && IMX_STORE_POLICY != 1) __builtin_nontemporal_store(x, p);
     else *p = x;
+}
+
+template <typename S, bool NT = true, bool W32 = false>
+__device__ __forceinline__ void st_pop(S *lat, uint64_t pstride, int i, uint32_t c, typename RealOf<S>::type v)
+{
+    st_ptr<S, NT>(pop_ptr<S, W32>(lat, pstride, i, c), WCLASS[i], v);
 }
 
 // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
@@ -225,6 +256,37 @@ __device__ __forceinline__ void collide_lp(float (&g)[Q], double pd, float u, fl
     }
 }
 
+// collide_lp for populations stored relative to the equilibrium of a free
+// stream (U, 0, 0) at p = S/3 (the open channel's inflow):
+// g = f - feq(S/3, U), and (pd, du, dv, dw) = (sum g, sum g e) are the
+// departures 3 p - S and u - U.  feq(p, u) - feq(S/3, U) =
+// w (pd - 1.5 (u.u - U^2) + 4.5 ((e.u)^2 - (e_x U)^2) + 3 e.(u - U)), written
+// in the departures so that FP32 resolves them whatever U is:
+// u.u - U^2 = du (du + 2U) + dv^2 + dw^2, (e.u)^2 - (e_x U)^2 =
+// (e.du) (e.du + 2 e_x U).  With U = 0 it is collide_lp.
+__device__ __forceinline__ void collide_lp_stream(float (&g)[Q], double pd, float du, float dv, float dw,
+                                                  float omega, float U)
+{
+#pragma clang fp contract(off)
+    const float uu = __builtin_fmaf(du, du + 2.f * U, __builtin_fmaf(dv, dv, dw * dw));
+    const float a = (float)pd - 1.5f * uu;
+    g[0] = __builtin_fmaf(omega, (float)WVAL[0] * a - g[0], g[0]);
+    const float U2 = 2.f * U;
+#pragma unroll
+    for (int k = 0; k < NPAIR; ++k) {
+        const int i = 2 * k + 1;
+        const float edu = edu_of<float>(i, du, dv, dw);
+        float ex = edu;
+        if (EX[i] > 0) ex = edu + U2;
+        if (EX[i] < 0) ex = edu - U2;
+        const float even = __builtin_fmaf(4.5f * edu, ex, a);
+        const float odd = 3.f * edu;
+        const float wi = (float)WVAL[WCLASS[i]];
+        g[i] = __builtin_fmaf(omega, wi * (even + odd) - g[i], g[i]);
+        g[i + 1] = __builtin_fmaf(omega, wi * (even - odd) - g[i + 1], g[i + 1]);
+    }
+}
+
 // Where a cell's populations come from and go to.
 struct CellCtx {
     uint32_t c;
@@ -309,9 +371,15 @@ __device__ __forceinline__ uint32_t src_off(const CellCtx &k, int i)
     return k.rowb[1 - EZ[i]][1 - EY[i]] + k.xv[1 - EX[i]];
 }
 
-template <typename S, int BC, int PAT, bool NTL = false>
+// AA storage reads and writes the same 27 addresses per cell: direction i is
+// stored where direction bb(i) was loaded from (even step: slot bb(i) of the
+// cell; odd step: slot i of x + e_i, which the load of bb(i) read from x -
+// e_bb(i)).  The gather hands its addresses to the scatter through `at`, so
+// the step keeps one set of addresses -- a scalar base and a 32-bit vector
+// offset each with W32 -- instead of forming a second set for the stores.
+template <typename S, int BC, int PAT, bool NTL = false, bool W32 = false>
 __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx &k,
-                                            typename RealOf<S>::type (&f)[Q])
+                                            typename RealOf<S>::type (&f)[Q], const S *(&at)[Q])
 {
     using R = typename RealOf<S>::type;
     constexpr bool F64 = std::is_same<S, double>::value;
@@ -321,7 +389,8 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
     if constexpr (PAT == PAT_AA_EVEN) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            f[i] = ld_pop<S, NTL>(src, ps, i, c);
+            at[i] = pop_ptr<const S, W32>(src, ps, i, c);
+            f[i] = ld_ptr<S, NTL>(at[i], WCLASS[i]);
             if constexpr (BC == BC_CAVITY) {
                 // the odd step stored the bounced population raw; the lid
                 // term is added on arrival, as in the pull form (a select:
@@ -338,8 +407,9 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             if constexpr (BC == BC_PERIODIC) {
-                if (NTL && EX[i] == 0) f[i] = ld_pop<S, true>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
-                else f[i] = ld_pop<S>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
+                at[i] = pop_ptr<const S, W32>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
+                if (NTL && EX[i] == 0) f[i] = ld_ptr<S, true>(at[i], WCLASS[i]);
+                else f[i] = ld_ptr<S, false>(at[i], WCLASS[i]);
             } else if constexpr (BC == BC_CAVITY) {
                 // Boundary choices are selects, not branches: a value loaded
                 // inside a divergent branch, or used by one, is waited for
@@ -348,10 +418,10 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
                 // plus the lid term on the moving wall.
                 const bool out = src_out(k, i);
                 const int s_in = PAT == PAT_PULL ? i : BB[i], s_out = PAT == PAT_PULL ? BB[i] : i;
-                const uint64_t at = out ? (uint64_t)s_out * ps + c : (uint64_t)s_in * ps + src_off(k, i);
+                at[i] = src + (out ? (uint64_t)s_out * ps + c : (uint64_t)s_in * ps + src_off(k, i));
                 R v;
-                if (NTL && EX[i] == 0) v = ld_at<S, true>(src, at, WCLASS[i]);
-                else v = ld_at<S>(src, at, WCLASS[i]);
+                if (NTL && EX[i] == 0) v = ld_ptr<S, true>(at[i], WCLASS[i]);
+                else v = ld_ptr<S>(at[i], WCLASS[i]);
                 if (EZ[i] < 0) {  // x - e_i outside whenever z is the top layer
                     R lid;
                     if constexpr (F64) lid = v + P.corr[i];
@@ -387,6 +457,12 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
                 if (i != 0 && k.m == 2) solid = *glb(P.mask + so) == 1;
                 const uint64_t at = solid ? (uint64_t)BB[i] * ps + c : (uint64_t)i * ps + so;
                 R v = ld_at<S>(src, at, WCLASS[i]);
+                if constexpr (!F64) {
+                    // shifted storage: g_i = g*_bb(i) + s_bb(i) - s_i (0 unless the
+                    // shift carries a free stream)
+                    const R b = v + (float)P.corr[i];
+                    v = solid ? b : v;
+                }
                 if (EX[i] > 0) {
                     R in;
                     if constexpr (F64) in = P.feq_in[i];
@@ -441,49 +517,70 @@ __device__ __forceinline__ void cell_collide(const FusedParams &P, const CellCtx
         double pd;
         float u, v, w;
         moments_lp(f, pd, u, v, w);
-        if (write_moments) {
-            *glb(P.mom_p + c) = (pd + P.shift) / 3.0;
-            *glb(P.mom_u + c) = u;
-            *glb(P.mom_v + c) = v;
-            *glb(P.mom_w + c) = w;
+        if constexpr (BC == BC_CHANNEL) {
+            // populations stored relative to the inflow equilibrium: the
+            // moments above are the departure from the free stream (U, 0, 0)
+            if (write_moments) {
+                *glb(P.mom_p + c) = (pd + P.shift) / 3.0;
+                *glb(P.mom_u + c) = (double)u + P.ush;
+                *glb(P.mom_v + c) = v;
+                *glb(P.mom_w + c) = w;
+            }
+            collide_lp_stream(f, pd, u, v, w, P.omega_f, (float)P.ush);
+        } else {
+            if (write_moments) {
+                *glb(P.mom_p + c) = (pd + P.shift) / 3.0;
+                *glb(P.mom_u + c) = u;
+                *glb(P.mom_v + c) = v;
+                *glb(P.mom_w + c) = w;
+            }
+            collide_lp(f, pd, u, v, w, P.omega_f);
         }
-        collide_lp(f, pd, u, v, w, P.omega_f);
     }
 }
 
-template <typename S, int BC, int PAT>
+// REUSE: the AA odd step stores through the gather's addresses (see
+// cell_gather); otherwise the store addresses are formed again from the
+// cell, which is cheaper in registers when the addresses of two cells per
+// thread, or the even step's in-place addresses, would be held.
+template <typename S, int BC, int PAT, bool W32 = false, bool REUSE = false>
 __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx &k,
-                                             const typename RealOf<S>::type (&f)[Q])
+                                             const typename RealOf<S>::type (&f)[Q], const S *const (&at)[Q])
 {
     S *dst = (S *)P.dst;
     const uint64_t ps = P.pstride;
     const uint32_t c = k.c;
     if constexpr (PAT == PAT_PULL) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) st_pop<S>(dst, ps, i, c, f[i]);
+        for (int i = 0; i < Q; ++i) st_pop<S, true, W32>(dst, ps, i, c, f[i]);
     } else if constexpr (PAT == PAT_AA_EVEN) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) st_pop<S>(dst, ps, BB[i], c, f[i]);
+        for (int i = 0; i < Q; ++i) st_pop<S, true, W32>(dst, ps, BB[i], c, f[i]);
     } else {
-        // to the natural slot of x + e_i, or bounced back into slot bb(i) at x
+        // to the natural slot of x + e_i, or bounced back into slot bb(i) at
+        // x: in both cases where direction bb(i) was loaded from.  Shifted
+        // writes (e_x != 0, not bounced) cover partial lines at row segment
+        // ends and stay plain stores, which merge in L2.
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            if constexpr (BC == BC_CAVITY) {
-                if (src_out(k, BB[i])) {
-                    st_pop<S>(dst, ps, BB[i], c, f[i]);
-                    continue;
-                }
-            }
-            // shifted writes cover partial lines at row segment ends
-            if (IMX_STORE_POLICY == 2 && EX[i] != 0) st_pop<S, false>(dst, ps, i, src_off(k, BB[i]), f[i]);
-            else st_pop<S>(dst, ps, i, src_off(k, BB[i]), f[i]);
+            bool bounced = false;
+            if constexpr (BC == BC_CAVITY) bounced = src_out(k, BB[i]);
+            const bool plain = IMX_STORE_POLICY == 2 && EX[i] != 0 && !bounced;
+            S *p;
+            if constexpr (REUSE) p = (S *)at[BB[i]];
+            else if constexpr (BC == BC_CAVITY)
+                p = bounced ? pop_ptr<S, W32>(dst, ps, BB[i], c) : pop_ptr<S, W32>(dst, ps, i, src_off(k, BB[i]));
+            else p = pop_ptr<S, W32>(dst, ps, i, src_off(k, BB[i]));
+            if (plain) st_ptr<S, false>(p, WCLASS[i], f[i]);
+            else st_ptr<S>(p, WCLASS[i], f[i]);
         }
     }
 }
 
 // CPT cells per thread, blockDim.x apart; the loads of all CPT cells are
 // issued before any of them is collided, to keep more bytes in flight.
-template <typename S, int BC, int PAT, int CPT, bool NTL>
+// W32: 32-bit offsets inside each population (see pop_ptr).
+template <typename S, int BC, int PAT, int CPT, bool NTL, bool W32>
 __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
 {
     using R = typename RealOf<S>::type;
@@ -492,24 +589,25 @@ __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
     const uint32_t base = b * blockDim.x * CPT + threadIdx.x;
     CellCtx k[CPT];
     R f[CPT][Q];
+    const S *at[CPT][Q];
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         cell_setup<BC>(P, base + (uint32_t)r * blockDim.x, k[r]);
-        if (k[r].live) cell_gather<S, BC, PAT, NTL>(P, k[r], f[r]);
+        if (k[r].live) cell_gather<S, BC, PAT, NTL, W32>(P, k[r], f[r], at[r]);
     }
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         if (!k[r].live) continue;
         cell_collide<S, BC>(P, k[r], f[r], P.mom_p != nullptr);
-        cell_scatter<S, BC, PAT>(P, k[r], f[r]);
+        cell_scatter<S, BC, PAT, W32, PAT == PAT_AA_ODD && CPT == 1>(P, k[r], f[r], at[r]);
     }
 }
 
-template <typename S, int CPT, bool NTL>
+template <typename S, int CPT, bool NTL, bool W32>
 hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
     dim3 grid(P.nblocks), blk(block);
-#define IMX_L(B, PT) hipLaunchKernelGGL((k_fused<S, B, PT, CPT, NTL>), grid, blk, 0, st, P)
+#define IMX_L(B, PT) hipLaunchKernelGGL((k_fused<S, B, PT, CPT, NTL, W32>), grid, blk, 0, st, P)
     if (bc == BC_PERIODIC) {
         if (pattern == PAT_PULL) IMX_L(BC_PERIODIC, PAT_PULL);
         else if (pattern == PAT_AA_EVEN) IMX_L(BC_PERIODIC, PAT_AA_EVEN);
@@ -534,15 +632,26 @@ hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block
     return hipGetLastError();
 }
 
+// Does every element offset inside one population fit the 32-bit byte
+// offsets of the W32 kernels?
+template <typename S>
+inline bool fits_w32(uint64_t pstride)
+{
+    return pstride * sizeof(S) <= (uint64_t(1) << 32);
+}
+
 template <typename S>
 hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
+    // lattices with populations beyond 4 GiB: 64-bit addresses, one cell
+    // per thread, plain loads (the same results)
+    if (!fits_w32<S>(P.pstride)) return launch_fused_cpt<S, 1, false, false>(bc, pattern, P, block, st);
     if (P.ntl) {
-        if (P.cpt == 2) return launch_fused_cpt<S, 2, true>(bc, pattern, P, block, st);
-        return launch_fused_cpt<S, 1, true>(bc, pattern, P, block, st);
+        if (P.cpt == 2) return launch_fused_cpt<S, 2, true, true>(bc, pattern, P, block, st);
+        return launch_fused_cpt<S, 1, true, true>(bc, pattern, P, block, st);
     }
-    if (P.cpt == 2) return launch_fused_cpt<S, 2, false>(bc, pattern, P, block, st);
-    return launch_fused_cpt<S, 1, false>(bc, pattern, P, block, st);
+    if (P.cpt == 2) return launch_fused_cpt<S, 2, false, true>(bc, pattern, P, block, st);
+    return launch_fused_cpt<S, 1, false, true>(bc, pattern, P, block, st);
 }
 
 }  // namespace imx

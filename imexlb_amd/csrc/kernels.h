@@ -69,8 +69,12 @@ struct FusedParams {
     const uint8_t *mask;     // channel: 0 fluid, 1 solid, 2 fluid next to solid
     double tau0;             // f <- f - (f - feq) / (tau0 + 0.5)
     float omega_f;           // 1 / (tau0 + 0.5) for the FP32 collision
-    double shift;            // reduced precision stores g = f - w_i * shift
-    double corr[Q];          // cavity: moving-lid term for a link crossing z = nz
+    double shift;            // reduced precision: S, with sum_i s_i = S (p = (sum g + S) / 3)
+    double ush;              // reduced-precision channel: the shift's free-stream x velocity U
+                             // (u = sum g e + U); 0 otherwise
+    double sh[Q];            // s_i (storage_shift), for the transfer and force kernels
+    double corr[Q];          // cavity: moving-lid term for a link crossing z = nz;
+                             // shifted channel: s_bb(i) - s_i, added to a link bounced off the solid
     double feq_in[Q];        // channel: inlet equilibrium, stored convention
     uint32_t nblocks;        // grid size (for the XCD-aware block order)
     int xcd_remap;           // 1: give each XCD one contiguous run of tiles
@@ -123,6 +127,24 @@ __host__ __device__ __forceinline__ bool faithful_sphere_solid(int glx, int x, i
     return (double)di * (double)di + (double)dj * (double)dj + (double)dk * (double)dk <= r * r;
 }
 
+// Storage shift of direction i: FP32/FP16 lattices store g_i = f_i - s_i.
+// The TGV and the cavity (U = 0) subtract s_i = w_i S, S = rho0; the open
+// channel subtracts its inflow equilibrium feq_i(S/3, (U, 0, 0)) in the
+// oracle's evaluation order, so that the stored values are the flow's
+// departure from the free stream and the inflow itself is stored as 0.
+inline void storage_shift(double S, double U, double out[Q])
+{
+    const double pin = S * (1.0 / 3.0);
+    for (int i = 0; i < Q; ++i) {
+        if (U == 0.0) {
+            out[i] = w(i) * S;
+        } else {
+            const double edu = (double)EX[i] * U, udu = U * U, eu2 = edu * edu;
+            out[i] = w(i) * pin * 3.0 + w(i) * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
+        }
+    }
+}
+
 // Launchers (defined next to their kernels).
 hipError_t launch_fused(int dtype, int bc, int pattern, const FusedParams &P, int block, hipStream_t st);
 
@@ -136,7 +158,7 @@ struct InitParams {
                              // 2 the faithful sphere (parabolic inlet profile, seeded perturbation)
     const double *tab;       // TGV: 9 tables [sx cx c2x | sy cy c2y | sz cz c2z]
     double u0, rho0, cs2, p0;
-    double shift;
+    double sh[Q];            // storage shift s_i
     const uint8_t *mask;
     uint64_t seed;           // kind 2
     double *mom[4];          // p, u, v, w of the initial fields (solid cells: 0);
@@ -155,7 +177,7 @@ struct XferParams {
     double *buf;             // [Q][nz][ny][nx]
     int layout;              // 0 natural, 1 swapped (AA even), 2 streamed (AA odd)
     int bc;                  // periodic or cavity (AA streamed layout needs the rule)
-    double shift;
+    double sh[Q];            // storage shift s_i
     double corr[Q];
     FastDiv divx, divy;
 };

@@ -83,3 +83,36 @@ def test_tgv512_reduced_precision_within_stated_error():
     e16, e32 = l2(vel["f16"], vel["f64"]), l2(vel["f32"], vel["f64"])
     print(f"[tgv512] velocity L2 vs FP64 after 200 steps: FP32 {e32:.3e}, FP16 {e16:.3e}", flush=True)
     assert e16 <= 1e-3 and e32 <= 1e-5
+
+
+def test_sphere_channel_fullsize_reduced_precision():
+    """The sphere-channel configuration at its own size (512x256x256, D 40 at
+    (128, 128, 128), inflow 0.05, Re 200) in FP32 and FP16 storage with FP32
+    collision against FP64 storage after 500 steps: the drag proxy and the
+    velocity field.  The FP32/FP16 lattices hold the departure from the
+    inflow equilibrium."""
+    from imexlb_amd import LBM
+    nx, ny, nz = 512, 256, 256
+    u_in, d, re = 0.05, 40.0, 200.0
+    tau = 3.0 * u_in * d / re
+    drag, vel = {}, {}
+    for dt in ("f64", "f32", "f16"):
+        print(f"[channel] {dt}: 500 steps", flush=True)
+        g = LBM(nx, ny, nz, tau, u0=u_in, case="sphere", mode="config", dtype=dt, storage="pull",
+                sphere=(128.0, 128.0, 128.0, d / 2))
+        g.Initialize()
+        g.step(500)
+        drag[dt] = g.sphere_force()
+        vel[dt] = g.moments()[1:]
+        g.close()
+
+    def l2(a, b):
+        return (sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b)) / sum(float(np.sum(y ** 2)) for y in b)) ** 0.5
+    err = {dt: (abs(drag[dt][0] - drag["f64"][0]) / abs(drag["f64"][0]), l2(vel[dt], vel["f64"]))
+           for dt in ("f32", "f16")}
+    print(f"[channel] after 500 steps: FP64 drag {drag['f64'][0]:.6e}; relative drag error / velocity L2 "
+          f"FP32 {err['f32'][0]:.3e} / {err['f32'][1]:.3e}, FP16 {err['f16'][0]:.3e} / {err['f16'][1]:.3e}",
+          flush=True)
+    assert drag["f64"][0] > 0
+    assert err["f32"][0] <= 1e-4 and err["f32"][1] <= 1e-4
+    assert err["f16"][0] <= 1e-2 and err["f16"][1] <= 1e-3
