@@ -88,6 +88,8 @@ SIGNATURES = [
     ("imexlb_get_f", _i, [_vp, _dp]),
     ("imexlb_set_f", _i, [_vp, _dp]),
     ("imexlb_diagnostics", _i, [_vp, _dp, _dp]),
+    # device pointers as integers: p, u, v, w, ncell, ke_mass, stream
+    ("imexlb_diagnostics_noalloc", _i, [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp]),
     ("imexlb_sphere_force", _i, [_vp, _dp]),
     ("imexlb_sync", _i, [_vp]),
     ("imexlb_get_info", _i, [_vp, ctypes.POINTER(Info)]),

@@ -38,6 +38,7 @@ SOURCES = [
     ("aux.hip", EXACT),
     ("phased.hip", EXACT),
     ("ensemble.hip", []),
+    ("noalloc.hip", EXACT),
     ("capi.cpp", EXACT),
     ("host.cpp", EXACT),
 ]

@@ -71,6 +71,18 @@ static int fail(imexlb_handle *h, int code, const char *fmt, ...)
     return code;
 }
 
+// The message imexlb_last_error(NULL) returns, for entry points that take no
+// handle (imexlb_diagnostics_noalloc).
+void imx::set_call_error(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_create_err = buf;
+}
+
 #define HIPCHK(h, expr)                                                                           \
     do {                                                                                          \
         hipError_t e_ = (expr);                                                                   \

@@ -192,6 +192,10 @@ hipError_t launch_force(int dtype, const FusedParams &P, double *partials, int n
                         double *out3, hipStream_t st);
 
 hipError_t launch_copy(const double *src, double *dst, size_t n, hipStream_t st);
+
+// Sets the message imexlb_last_error(NULL) returns on this thread (entry
+// points without a handle).
+void set_call_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 // Bandwidth probe `kind` (imexlb_bandwidth_probe) over n doubles; returns the
 // number of streams it moves (2 for copies, 1 for read- or write-only), 0 for
 // an unknown kind.

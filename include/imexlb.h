@@ -176,8 +176,31 @@ int imexlb_diagnostics(imexlb_handle *h, double *ke, double *mass);
  * post-collision populations (drag proxy), FP64. */
 int imexlb_sphere_force(imexlb_handle *h, double *force3);
 
+/* Allocation-free form of imexlb_diagnostics.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  It takes no handle: p, u, v, w are caller-owned device arrays
+ * of ncell FP64 values each (as the moments of imexlb_get_moments), and
+ * ke_mass is a caller-owned device array of 2 doubles that receives
+ * 0.5 sum |u|^2 and sum 3p -- the same bits imexlb_diagnostics returns for
+ * the same fields.  It enqueues two kernels on `stream` (a hipStream_t; NULL
+ * = the null stream) of the current device and returns without waiting;
+ * nothing it does allocates, synchronises or clears memory, so it may be
+ * captured into a HIP graph.  Scratch: a fixed 16 KiB device array held by
+ * the library on every GPU it runs on, until the process ends; calls that
+ * overlap on one GPU (on different streams, or replays of graphs holding
+ * it) race on it and must be ordered by the caller.  The first call on a
+ * device looks the array's address up (make it outside a capture if the
+ * capture must hold kernels only).  ncell in [0, IMEXLB_NOALLOC_MAX_CELLS]
+ * (2^31), else IMEXLB_EINVAL; imexlb_last_error(NULL) gives the message. */
+#define IMEXLB_NOALLOC_MAX_CELLS 2147483648LL
+int imexlb_diagnostics_noalloc(const double *p, const double *u, const double *v, const double *w,
+                               int64_t ncell, double *ke_mass, void *stream);
+
 int imexlb_sync(imexlb_handle *h);
 int imexlb_get_info(imexlb_handle *h, imexlb_info *info);
+/* Message of the last failure on h; with h = NULL, of the last failed
+ * imexlb_create or handle-free call on this thread. */
 const char *imexlb_last_error(const imexlb_handle *h);
 const char *imexlb_strerror(int status);
 
