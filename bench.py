@@ -7,7 +7,7 @@ the FP64 (drop-in) throughput in GLUPS; the same run also steps FP32- and
 FP16-shifted storage from the same initial state and reports their GLUPS and
 their velocity L2 error against the FP64 run after the same steps.
 
-    python bench.py [--steps K] [--warmup W] [--n 512] [--storage aa|pull] [--placements P]
+    python bench.py [--steps K] [--warmup W] [--n 512] [--storage aa|pull] [--placement window|allocations|none]
 
 Prints one JSON line.  Inputs are resident in HBM before the timed region; the
 timed region is K launches on the library's stream, bracketed by HIP events.
@@ -36,7 +36,9 @@ def tgv_tau(n, re, u0):
     return u0 * n / re / (1.0 / 3.0) / 2 / PI_R
 
 
-def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None, placements=1):
+def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None, placement=None):
+    """placement: None (the first allocation), ("allocations", trials) or
+    ("window", slack_gib, step_gib)."""
     from imexlb_amd import LBM
     kw = {}
     if launch:
@@ -45,35 +47,49 @@ def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None, pl
         tune = False
     g = LBM(n, n, n, tgv_tau(n, re, u0), u0=u0, case="tgv", engine="fused", dtype=dtype,
             storage=storage, **kw)
+    g.Initialize()
+    info_p = None
+    lat_gb = 27 * n ** 3 * {"f64": 8, "f32": 4, "f16": 2}[dtype] * (2 if storage == "pull" else 1) / 1e9
+    if placement and placement[0] == "allocations" and placement[1] > 1:
+        # the lattice's HBM placement moves the step rate by up to 20 %; the
+        # library times up to `trials` allocations and keeps the fastest
+        kept, ms_each = g.tune_placement(placement[1], steps=10)
+        rates = [round(n ** 3 / (m * 1e-3) / 1e9, 3) if m > 0 else None for m in ms_each]
+        info_p = {"method": "separate allocations", "trials": placement[1], "glups_each": rates, "kept": kept,
+                  "held_gb_during_search": round(placement[1] * lat_gb, 1), "held_gb_after": round(lat_gb, 1)}
+        log(f"[bench] {dtype}: placement GLUPS {rates}, kept {kept}")
+    elif placement and placement[0] == "window":
+        # the rate follows the lattice's base address smoothly: slide it
+        # through one allocation of lattice + slack and keep the fastest
+        # offset (the handle holds the window)
+        _, slack, step = placement
+        kept, ms_each = g.tune_placement_window(slack_gib=slack, step_gib=step, steps=10)
+        rates = [round(n ** 3 / (m * 1e-3) / 1e9, 3) if m > 0 else None for m in ms_each]
+        held = g.info()["device_bytes"] / 1e9
+        info_p = {"method": "window", "slack_gib": slack, "step_gib": step, "glups_each": rates, "kept": kept,
+                  "held_gb_during_search": round(lat_gb + slack * 2 ** 30 / 1e9, 1),
+                  "held_gb_after": round(held, 1)}
+        log(f"[bench] {dtype}: window GLUPS {rates}, kept offset {kept} ({rates[kept]})")
     if tune:
-        # launch shape from the in-project autotuner (re-initialises the state)
+        # launch shape from the in-project autotuner, on the placement kept
+        # above (re-initialises the state)
         choice = g.autotune(os.environ.get("IMEXLB_TUNE_CACHE"), steps=8)
         log(f"[bench] {dtype}: autotuned block {choice[0]}, cells/thread {choice[1]}, xcd order {choice[2]}, "
             f"nt loads {choice[3]}")
-    else:
-        g.Initialize()
-    placement = None
-    if placements > 1:
-        # the lattice's HBM placement moves the step rate by several percent;
-        # the library times up to `placements` allocations and keeps the
-        # fastest (re-initialises the state)
-        kept, ms_each = g.tune_placement(placements, steps=10)
-        rates = [round(n ** 3 / (m * 1e-3) / 1e9, 3) if m > 0 else None for m in ms_each]
-        placement = {"trials": placements, "glups_each": rates, "kept": kept}
-        log(f"[bench] {dtype}: placement GLUPS {rates}, kept {kept}")
     g.step(warmup, moments_every=0)
     g.sync()
     ms = g.step_timed(steps, moments_every=0)
     # one more step writes the moments used for the error report
     g.step(1, moments_every=1)
     info = g.info()
-    info["placement"] = placement
+    info["placement"] = info_p
     return g, ms, info
 
 
 def placement_summary(placement, bytes_lup):
-    """First-allocation, median and kept rates of a placement search, as
-    GLUPS and as the fraction of the 8 TB/s peak."""
+    """First, median and kept rates of a placement search, as GLUPS and as
+    the fraction of the 8 TB/s peak.  "First" is the first allocation, or the
+    window's first offset (where a fresh allocation of the lattice lands)."""
     if not placement:
         return {}
     rates = [r for r in placement["glups_each"] if r]
@@ -81,6 +97,19 @@ def placement_summary(placement, bytes_lup):
     first, med, kept = rates[0], float(np.median(rates)), placement["glups_each"][placement["kept"]]
     return {"glups_first_alloc": first, "glups_median": med, "glups_kept": kept,
             "frac_first_alloc": frac(first), "frac_median": frac(med), "frac_kept": frac(kept)}
+
+
+def value_basis(p):
+    if not p:
+        return "value and frac: the timed run on the first lattice allocation"
+    if p["method"] == "window":
+        return (f"value and frac: the timed run with the lattice at the fastest of {len(p['glups_each'])} offsets "
+                f"({p['step_gib']} GiB apart) in one window of lattice + {p['slack_gib']} GiB "
+                f"({p['held_gb_during_search']} GB held during the search, {p['held_gb_after']} GB after); "
+                "glups/frac_first_alloc is offset 0, glups/frac_median the median offset")
+    return (f"value and frac: the timed run on the fastest of {p['trials']} lattice allocations "
+            f"({p['held_gb_during_search']} GB held during the search); glups/frac_first_alloc is the first "
+            "allocation, glups/frac_median the median candidate")
 
 
 def trials_for_budget(n, dtype, storage, default, budget_gb):
@@ -220,12 +249,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lowp", action="store_true")
-    ap.add_argument("--placements", type=int, default=0,
-                    help="lattice allocations timed before the run, fastest kept (1: first only; "
-                         "0: up to 8 / 8 / 12 for FP64 / FP32 / FP16, as many as --placement-budget-gb holds)")
+    ap.add_argument("--placement", default="window", choices=["window", "allocations", "none"],
+                    help="lattice placement search: offsets in one window of device memory chunks (default), "
+                         "separate allocations, or the first allocation")
+    ap.add_argument("--window-slack-gib", type=float, default=64.0,
+                    help="window beyond the lattice for --placement window (held during the search only)")
+    ap.add_argument("--placements", type=int, default=None,
+                    help="time this many separate allocations instead (1: the first allocation only)")
     ap.add_argument("--placement-budget-gb", type=float, default=128.0,
-                    help="device memory the placement search may hold at once (all candidates live "
-                         "until it ends): 4 FP64, 8 FP32 and 12 FP16 lattices at 512^3 AA")
+                    help="--placement allocations: device memory the candidates may hold at once")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -240,15 +272,18 @@ def main():
     cells = n ** 3
     log(f"[bench] TGV {n}^3 {a.storage} FP64: warmup {W}, timed {K}")
     launch = tuple(int(v) for v in a.launch.split(",")) if a.launch else None
-    trials, held_gb = {}, {}
+    placement = {}
     for dt in ("f64", "f32", "f16"):
-        t, lat_gb = trials_for_budget(n, dt, a.storage, a.placements or {"f64": 8, "f32": 8, "f16": 12}[dt],
-                                      a.placement_budget_gb)
-        trials[dt] = t
-        held_gb[dt] = round(t * lat_gb, 1)
-    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune, launch, trials["f64"])
-    if info64["placement"]:
-        info64["placement"]["held_gb"] = held_gb["f64"]
+        if a.placements is not None:
+            placement[dt] = ("allocations", a.placements) if a.placements > 1 else None
+        elif a.placement == "window":
+            placement[dt] = ("window", a.window_slack_gib, {"f64": 2, "f32": 1, "f16": 1}[dt])
+        elif a.placement == "allocations":
+            t, _ = trials_for_budget(n, dt, a.storage, {"f64": 8, "f32": 8, "f16": 12}[dt], a.placement_budget_gb)
+            placement[dt] = ("allocations", t)
+        else:
+            placement[dt] = None
+    g64, ms64, info64 = run_gpu(n, K, W, a.storage, "f64", a.u0, a.re, not a.no_tune, launch, placement["f64"])
     glups64 = cells * K / (ms64 * 1e-3) / 1e9
     per_launch_ms = ms64 / K
     bytes_lup = info64["bytes_per_lup"]
@@ -263,10 +298,9 @@ def main():
     if not a.no_lowp:
         for dt in ("f32", "f16"):
             log(f"[bench] TGV {n}^3 {a.storage} {dt}")
-            g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re, not a.no_tune, None, trials[dt])
+            g, ms, info = run_gpu(n, K, W, a.storage, dt, a.u0, a.re, not a.no_tune, None, placement[dt])
             gl = cells * K / (ms * 1e-3) / 1e9
             if info["placement"]:
-                info["placement"]["held_gb"] = held_gb[dt]
                 info["placement"].update(placement_summary(info["placement"], info["bytes_per_lup"]))
             m = g.moments()
             err = vel_l2(m, m64)
@@ -326,11 +360,7 @@ def main():
                      "bandwidth_probes_GBps": {k: round(v, 1) for k, v in probes.items()},
                      "placement": info64["placement"],
                      **placement_summary(info64["placement"], bytes_lup),
-                     "value_basis": ("value and frac: the timed run on the fastest of "
-                                     f"{trials['f64']} lattice allocations ({held_gb['f64']} GB held during the "
-                                     "search); glups/frac_first_alloc is the first allocation, "
-                                     "glups/frac_median the median candidate" if trials["f64"] > 1 else
-                                     "value and frac: the timed run on the first lattice allocation")},
+                     "value_basis": value_basis(info64["placement"])},
         "cpu_baseline": cpu,
         "tgv_error": {"after_steps": W + K + 1, "f64_vel_l2_vs_analytic": an64,
                       "note": "analytic = initial TGV field decayed by exp(-3 nu k^2 t), the linear solution; "

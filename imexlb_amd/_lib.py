@@ -105,6 +105,8 @@ SIGNATURES = [
     ("imexlb_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_set_launch", _i, [_vp, _i, _i, _i, _i]),
     ("imexlb_tune_placement", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)]),
+    ("imexlb_tune_placement_window", _i, [_vp, ctypes.c_int64, ctypes.c_int64, _i, _fp, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_ensemble_generate", _i, [ctypes.c_uint64, _i, ctypes.POINTER(EnsembleDomain)]),
     ("imexlb_ensemble_create", _i, [ctypes.POINTER(EnsembleDomain), _i, _i, _i, _i, ctypes.POINTER(_vp)]),
     ("imexlb_ensemble_destroy", _i, [_vp]),
@@ -118,6 +120,8 @@ SIGNATURES = [
     ("imexlb_ensemble_autotune", _i, [_vp, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_ensemble_tune_placement", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float),
                                             ctypes.POINTER(ctypes.c_int32)]),
+    ("imexlb_ensemble_tune_placement_window", _i, [_vp, ctypes.c_int64, ctypes.c_int64, _i, _fp, ctypes.c_int32,
+                                                   ctypes.POINTER(ctypes.c_int32)]),
     ("imexlb_ensemble_last_error", ctypes.c_char_p, [_vp]),
 ]
 

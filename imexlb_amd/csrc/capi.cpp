@@ -25,6 +25,10 @@ struct imexlb_handle {
     bool own_stream = false;
     // fused engine
     void *lat[2] = {nullptr, nullptr};
+    // set by imexlb_tune_placement_window: lat[] lie in this one allocation,
+    // of window_bytes bytes (lattice(s) plus the search's slack)
+    void *window = nullptr;
+    size_t window_bytes = 0;
     int cur = 0;             // pull: lattice holding the current state
     double *mom[4] = {nullptr, nullptr, nullptr, nullptr};
     uint8_t *mask = nullptr;
@@ -430,9 +434,6 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
             const double eu2 = edu * edu;
             const double feq = w(i) * pin * 3.0 + w(i) * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
             h->feq_in[i] = c.dtype == IMEXLB_F64 ? feq : feq - h->sh[i];
-            // a link bounced off the ball arrives in the other direction's
-            // stored convention
-            h->corr[i] = c.dtype == IMEXLB_F64 ? 0.0 : h->sh[BB[i]] - h->sh[i];
         }
         // mask: 1 solid, 2 fluid with a solid neighbour (y, z periodic)
         std::vector<uint8_t> m((size_t)h->ncell, 0);
@@ -496,8 +497,10 @@ int imexlb_destroy(imexlb_handle *h)
     if (!h) return IMEXLB_OK;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (auto &p : h->lat)
-        if (p) hipFree(p);
+    if (h->window) hipFree(h->window);
+    else
+        for (auto &p : h->lat)
+            if (p) hipFree(p);
     for (auto &p : h->mom)
         if (p) hipFree(p);
     if (h->mask) hipFree(h->mask);
@@ -1001,6 +1004,11 @@ int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per
     steps += steps & 1;  // whole AA periods
     const size_t lat_bytes = (size_t)Q * h->pstride * h->elem;
     const int nlat = h->cfg.storage == IMEXLB_PULL ? 2 : 1;
+    // candidate 0 is the current lattice, which may lie in a window
+    void *const win0 = h->window;
+    const size_t win0_bytes = h->window_bytes;
+    h->window = nullptr;
+    h->window_bytes = 0;
     std::vector<std::array<void *, 2>> cand{{h->lat[0], h->lat[1]}};
     std::vector<float> ms(trials, 0.0f);
     auto time_current = [&](float &out) -> int {
@@ -1033,18 +1041,98 @@ int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per
         for (int t = 1; t < (int)cand.size(); ++t)
             if (ms[t] < ms[best]) best = t;
     (void)hipStreamSynchronize(h->stream);
-    for (int t = 0; t < (int)cand.size(); ++t)
-        if (t != best)
-            for (void *q : cand[t])
-                if (q) hipFree(q);
+    for (int t = 0; t < (int)cand.size(); ++t) {
+        if (t == best) continue;
+        if (t == 0 && win0) {
+            h->device_bytes -= (int64_t)win0_bytes - (int64_t)(nlat * lat_bytes);
+            hipFree(win0);
+            continue;
+        }
+        for (void *q : cand[t])
+            if (q) hipFree(q);
+    }
     h->lat[0] = cand[best][0];
     h->lat[1] = cand[best][1];
+    if (best == 0) {
+        h->window = win0;
+        h->window_bytes = win0_bytes;
+    }
     const int rc2 = imexlb_initialize(h);
     if (rc) return rc;
     if (ms_per_step)
         for (int t = 0; t < trials; ++t) ms_per_step[t] = ms[t];
     if (chosen) *chosen = best;
     return rc2;
+}
+
+int imexlb_tune_placement_window(imexlb_handle *h, int64_t slack_bytes, int64_t step_bytes, int steps,
+                                 float *ms_per_pos, int32_t npos_max, int32_t *chosen)
+{
+    if (!h) return IMEXLB_EINVAL;
+    if (h->ph) return fail(h, IMEXLB_EUNSUPPORTED, "the phased engine keeps its own layout");
+    const int64_t two_mib = 2ll << 20;
+    if (step_bytes < two_mib || step_bytes % two_mib || slack_bytes < 0 || slack_bytes % step_bytes)
+        return fail(h, IMEXLB_EINVAL, "window step must be a multiple of 2 MiB dividing the slack");
+    const int64_t npos = slack_bytes / step_bytes + 1;
+    if (npos > 256 || npos > npos_max)
+        return fail(h, IMEXLB_EINVAL, "%lld window positions (at most %d)", (long long)npos,
+                    std::min(256, (int)npos_max));
+    int rc = set_device(h);
+    if (rc) return rc;
+    if (steps < 2) steps = 2;
+    steps += steps & 1;  // whole AA periods
+    const size_t lat_bytes = (size_t)Q * h->pstride * h->elem;
+    const int nlat = h->cfg.storage == IMEXLB_PULL ? 2 : 1;
+    const size_t total = (size_t)nlat * lat_bytes + (size_t)slack_bytes;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(h, hipMemGetInfo(&free_b, &total_b));
+    if (free_b < total + (2ull << 30))
+        return fail(h, IMEXLB_ENOMEM, "window of %zu bytes does not fit (%zu free)", total, free_b);
+    // one ordinary allocation, kept whole (giving the slack back would need
+    // memory mapped in chunks and released after kernels used it)
+    void *big = nullptr;
+    if (hipMalloc(&big, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(h, IMEXLB_ENOMEM, "window of %zu bytes", total);
+    }
+    void *old_lat[2] = {h->lat[0], h->lat[1]};
+    std::vector<float> ms((size_t)npos, 0.0f);
+    for (int64_t k = 0; k < npos && !rc; ++k) {
+        h->lat[0] = (char *)big + k * step_bytes;
+        h->lat[1] = nlat == 2 ? (char *)h->lat[0] + lat_bytes : nullptr;
+        rc = imexlb_initialize(h);
+        if (!rc) rc = imexlb_step(h, 2, 0);
+        if (!rc) rc = imexlb_step_timed(h, steps, 0, &ms[k]);
+        if (!rc) ms[k] /= (float)steps;
+    }
+    (void)hipStreamSynchronize(h->stream);
+    if (rc) {  // keep the lattice the handle had
+        h->lat[0] = old_lat[0];
+        h->lat[1] = old_lat[1];
+        hipFree(big);
+        const int rc2 = imexlb_initialize(h);
+        return rc ? rc : rc2;
+    }
+    int64_t best = 0;
+    for (int64_t k = 1; k < npos; ++k)
+        if (ms[k] < ms[best]) best = k;
+    h->lat[0] = (char *)big + best * step_bytes;
+    h->lat[1] = nlat == 2 ? (char *)h->lat[0] + lat_bytes : nullptr;
+    if (h->window) {
+        h->device_bytes -= (int64_t)h->window_bytes;
+        hipFree(h->window);
+    } else {
+        h->device_bytes -= (int64_t)(nlat * lat_bytes);
+        for (void *q : old_lat)
+            if (q) hipFree(q);
+    }
+    h->window = big;
+    h->window_bytes = total;
+    h->device_bytes += (int64_t)total;
+    if (ms_per_pos)
+        for (int64_t k = 0; k < npos; ++k) ms_per_pos[k] = ms[k];
+    if (chosen) *chosen = (int32_t)best;
+    return imexlb_initialize(h);
 }
 
 // Times `reps` launches of bandwidth probe `kind` (-1: the plain copy of

@@ -120,6 +120,8 @@ struct imexlb_ensemble {
     std::vector<uint2> tiles;
     uint2 *dtiles = nullptr;
     void *pool = nullptr;                // every lattice, in one allocation
+    void *pool_alloc = nullptr;          // the allocation pool lies in when a window search placed it
+    size_t window_bytes = 0;             // its size (pool_bytes + slack)
     size_t pool_bytes = 0;
     std::vector<size_t> lat_off;         // offset of each lattice in the pool
     std::vector<void *> lat;             // 2 per domain (second null for AA)
@@ -311,7 +313,8 @@ int imexlb_ensemble_destroy(imexlb_ensemble *e)
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
-    if (e->pool) (void)hipFree(e->pool);
+    if (e->pool_alloc) (void)hipFree(e->pool_alloc);
+    else if (e->pool) (void)hipFree(e->pool);
     for (double *p : e->mom)
         if (p) (void)hipFree(p);
     for (double *p : e->tabs)
@@ -601,6 +604,7 @@ int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, fl
     if (steps < 2) steps = 2;
     steps += steps & 1;
     std::vector<void *> cand{e->pool};
+    void *const alloc0 = e->pool_alloc;  // candidate 0 may lie in a window
     std::vector<float> ms(trials, 0.0f);
     auto time_current = [&](float &out) -> int {
         int r = imexlb_ensemble_initialize(e);
@@ -628,7 +632,12 @@ int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, fl
             if (ms[t] < ms[best]) best = t;
     (void)hipStreamSynchronize(e->stream);
     for (int t = 0; t < (int)cand.size(); ++t)
-        if (t != best) (void)hipFree(cand[t]);
+        if (t != best) (void)hipFree(t == 0 && alloc0 ? alloc0 : cand[t]);
+    if (best != 0 && alloc0) {
+        e->device_bytes -= (int64_t)(e->window_bytes - e->pool_bytes);
+        e->pool_alloc = nullptr;
+        e->window_bytes = 0;
+    }
     int rc2 = use_pool(e, cand[best]);
     if (!rc2) rc2 = imexlb_ensemble_initialize(e);
     if (rc) return rc;
@@ -636,6 +645,63 @@ int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, fl
         for (int t = 0; t < trials; ++t) ms_per_step[t] = ms[t];
     if (chosen) *chosen = best;
     return rc2;
+}
+
+// As imexlb_tune_placement_window, for the pool that holds every member's
+// lattices: the pool slides through one allocation of pool + slack.
+int imexlb_ensemble_tune_placement_window(imexlb_ensemble *e, int64_t slack_bytes, int64_t step_bytes, int steps,
+                                          float *ms_per_pos, int32_t npos_max, int32_t *chosen)
+{
+    if (!e) return IMEXLB_EINVAL;
+    const int64_t two_mib = 2ll << 20;
+    if (step_bytes < two_mib || step_bytes % two_mib || slack_bytes < 0 || slack_bytes % step_bytes)
+        return efail(e, IMEXLB_EINVAL, "window step must be a multiple of 2 MiB dividing the slack");
+    const int64_t npos = slack_bytes / step_bytes + 1;
+    if (npos > 256 || npos > npos_max)
+        return efail(e, IMEXLB_EINVAL, "%lld window positions (at most %d)", (long long)npos,
+                     std::min(256, (int)npos_max));
+    ECHK(e, hipSetDevice(e->device));
+    if (steps < 2) steps = 2;
+    steps += steps & 1;
+    const size_t total = e->pool_bytes + (size_t)slack_bytes;
+    size_t free_b = 0, total_b = 0;
+    ECHK(e, hipMemGetInfo(&free_b, &total_b));
+    if (free_b < total + (2ull << 30))
+        return efail(e, IMEXLB_ENOMEM, "window of %zu bytes does not fit (%zu free)", total, free_b);
+    void *big = nullptr;
+    if (hipMalloc(&big, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return efail(e, IMEXLB_ENOMEM, "window of %zu bytes", total);
+    }
+    void *const old_pool = e->pool, *const old_alloc = e->pool_alloc;
+    std::vector<float> ms((size_t)npos, 0.0f);
+    int rc = IMEXLB_OK;
+    for (int64_t k = 0; k < npos && !rc; ++k) {
+        rc = use_pool(e, (char *)big + k * step_bytes);
+        if (!rc) rc = imexlb_ensemble_initialize(e);
+        if (!rc) rc = imexlb_ensemble_step(e, 2, IMEXLB_ENS_GRAPH, 0);
+        if (!rc) rc = imexlb_ensemble_step_timed(e, steps, IMEXLB_ENS_GRAPH, 0, &ms[k]);
+        if (!rc) ms[k] /= (float)steps;
+    }
+    (void)hipStreamSynchronize(e->stream);
+    if (rc) {  // keep the pool the ensemble had
+        const int rc1 = use_pool(e, old_pool);
+        (void)hipFree(big);
+        const int rc2 = rc1 ? rc1 : imexlb_ensemble_initialize(e);
+        return rc ? rc : rc2;
+    }
+    int64_t best = 0;
+    for (int64_t k = 1; k < npos; ++k)
+        if (ms[k] < ms[best]) best = k;
+    (void)hipFree(old_alloc ? old_alloc : old_pool);
+    e->device_bytes += (int64_t)slack_bytes - (int64_t)(old_alloc ? e->window_bytes - e->pool_bytes : 0);
+    e->pool_alloc = big;
+    e->window_bytes = total;
+    rc = use_pool(e, (char *)big + best * step_bytes);
+    if (ms_per_pos)
+        for (int64_t k = 0; k < npos; ++k) ms_per_pos[k] = ms[k];
+    if (chosen) *chosen = (int32_t)best;
+    return rc ? rc : imexlb_ensemble_initialize(e);
 }
 
 const char *imexlb_ensemble_last_error(const imexlb_ensemble *e) { return e ? e->err.c_str() : ""; }

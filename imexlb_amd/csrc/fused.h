@@ -458,10 +458,14 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
                 const uint64_t at = solid ? (uint64_t)BB[i] * ps + c : (uint64_t)i * ps + so;
                 R v = ld_at<S>(src, at, WCLASS[i]);
                 if constexpr (!F64) {
-                    // shifted storage: g_i = g*_bb(i) + s_bb(i) - s_i (0 unless the
-                    // shift carries a free stream)
-                    const R b = v + (float)P.corr[i];
-                    v = solid ? b : v;
+                    // shifted storage: a link bounced off the ball arrives as
+                    // g_i = g*_bb(i) + s_bb(i) - s_i = g*_bb(i) - 6 w_i e_x U (the
+                    // inflow equilibrium's stream U runs along x: nothing to add
+                    // where e_x = 0)
+                    if (EX[i] != 0) {
+                        const R b = v - (float)(WVAL[WCLASS[i]] * EX[i]) * (6.f * (float)P.ush);
+                        v = solid ? b : v;
+                    }
                 }
                 if (EX[i] > 0) {
                     R in;

@@ -73,8 +73,7 @@ struct FusedParams {
     double ush;              // reduced-precision channel: the shift's free-stream x velocity U
                              // (u = sum g e + U); 0 otherwise
     double sh[Q];            // s_i (storage_shift), for the transfer and force kernels
-    double corr[Q];          // cavity: moving-lid term for a link crossing z = nz;
-                             // shifted channel: s_bb(i) - s_i, added to a link bounced off the solid
+    double corr[Q];          // cavity: moving-lid term for a link crossing z = nz
     double feq_in[Q];        // channel: inlet equilibrium, stored convention
     uint32_t nblocks;        // grid size (for the XCD-aware block order)
     int xcd_remap;           // 1: give each XCD one contiguous run of tiles

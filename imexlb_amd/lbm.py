@@ -182,6 +182,19 @@ class LBM:
         self._chk(lib().imexlb_tune_placement(self.h, int(trials), int(steps), ms, ctypes.byref(ch)))
         return ch.value, list(ms)
 
+    def tune_placement_window(self, slack_gib=32, step_gib=4, steps=10):
+        """Time the step with the lattice at offsets 0, step, 2 step .. slack
+        inside one allocation of the lattice plus `slack_gib` GiB and keep the
+        fastest; the handle then holds the whole window.  Returns (index
+        kept, [ms per step at each offset])."""
+        step = int(step_gib * (1 << 30))
+        slack = int(slack_gib * (1 << 30)) // step * step
+        n = slack // step + 1
+        ms = (ctypes.c_float * n)()
+        ch = ctypes.c_int32()
+        self._chk(lib().imexlb_tune_placement_window(self.h, slack, step, int(steps), ms, n, ctypes.byref(ch)))
+        return ch.value, list(ms)
+
     def info(self):
         i = L.Info()
         self._chk(lib().imexlb_get_info(self.h, ctypes.byref(i)))
@@ -320,6 +333,18 @@ class Ensemble:
         ms = (ctypes.c_float * int(trials))()
         ch = ctypes.c_int32()
         self._chk(lib().imexlb_ensemble_tune_placement(self.h, int(trials), int(steps), ms, ctypes.byref(ch)))
+        return ch.value, list(ms)
+
+    def tune_placement_window(self, slack_gib=32, step_gib=1, steps=6):
+        """As LBM.tune_placement_window, for the allocation that holds every
+        member's lattices; the ensemble is reset to its initial state."""
+        step = int(step_gib * (1 << 30))
+        slack = int(slack_gib * (1 << 30)) // step * step
+        n = slack // step + 1
+        ms = (ctypes.c_float * n)()
+        ch = ctypes.c_int32()
+        self._chk(lib().imexlb_ensemble_tune_placement_window(self.h, slack, step, int(steps), ms, n,
+                                                              ctypes.byref(ch)))
         return ch.value, list(ms)
 
     def info(self):

@@ -2,12 +2,13 @@
 line.
 
     python -m imexlb_amd.runs precision [--n 512] [--steps 5000]
-    python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000] [--placements 8]
-    python -m imexlb_amd.runs sphere    [--steps 50000] [--placements 8]
-    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements 8]
+    python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000] [--placements N]
+    python -m imexlb_amd.runs sphere    [--steps 50000] [--placements N]
+    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements N]
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
     python -m imexlb_amd.runs phased    [--n 256] [--steps 20]
+    python -m imexlb_amd.runs placement [--n 512] [--storage aa]
 
 precision: TGV 512^3, U0 0.05, Re 1600, AA storage in FP64 / FP32 / FP16
     (shifted); kinetic energy against the analytic early-time decay
@@ -21,9 +22,17 @@ ensemble: 256 domains from seed 20261202 (TGV / cavity, edges 32..96,
     Re 50..1000, lid 0.05..0.1), FP32 shifted storage; persistent launch with
     graph replay vs persistent launches vs one launch per domain; per-domain
     velocity error against the same ensemble in FP64.
+Every run but timing places its lattices with the window search (the
+fastest offset of the lattice inside one allocation of lattice + 32-64 GiB);
+--placements N times N separate allocations instead, --placements 1 keeps the
+first.
 timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, a fixed launch
     shape (or the autotuner's), `steps` timed steps after 4 warm-up steps;
     GLUPS and algorithmic GB/s.  The short run to profile one kernel with.
+placement: TGV n^3 in each storage type at a fixed launch shape: the step
+    rate with the lattice at every offset of a window search (twice, in two
+    separate windows: 64 GiB in 2 GiB steps for FP64, 1 GiB steps for FP32 and
+    FP16), then over four separate allocations.
 phased: the reference's own boundary code (faithful mode) on the phased
     engine's multi-step path: the cavity at n^3 (lid 0.1, tau0 0.096) and the
     sphere input of sphere/input.in (512 x 128 x 128, Re 100, u0 0.05); and the
@@ -51,6 +60,32 @@ def l2(a, b):
     num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
     den = sum(float(np.sum(y ** 2)) for y in b)
     return math.sqrt(num / den) if den > 0 else float("nan")
+
+
+def place(g, a, cells, label):
+    """Lattice placement search before a timed run: with --placements N > 1,
+    the fastest of N separate allocations; by default the fastest offset of
+    a window search (the lattice slid through one allocation of lattice + 64
+    GiB, 32 GiB for lattices under 16 GiB, which the handle keeps).
+    --placements 1: the first allocation.  Returns the record, None if no
+    search ran."""
+    def rate(m):
+        return round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None
+    if a.placements is not None:
+        if a.placements <= 1:
+            return None
+        kept, ms = g.tune_placement(a.placements, steps=10)
+        rec = {"method": "separate allocations", "kept": kept, "glups_each": [rate(m) for m in ms]}
+    else:
+        lat_gib = g.info()["device_bytes"] / 2 ** 30
+        step = 2 if lat_gib >= 16 else (1 if lat_gib >= 4 else 0.5)
+        slack = 64 if lat_gib >= 16 else 32
+        kept, ms = g.tune_placement_window(slack_gib=slack, step_gib=step, steps=10)
+        rec = {"method": "window", "slack_gib": slack, "step_gib": step, "kept": kept,
+               "glups_each": [rate(m) for m in ms]}
+    log(f"[{label}] placement: {rec['method']}, kept {rec['kept']} ({rec['glups_each'][rec['kept']]} GLUPS), "
+        f"range {min(r for r in rec['glups_each'] if r)}-{max(rec['glups_each'])}")
+    return rec
 
 
 def precision(a):
@@ -106,14 +141,10 @@ def cavity(a):
         "phased": LBM(n, n, n, tau, u0=0.0, case="cavity", mode="config", engine="phased", lid_u=lid),
     }
     out = {"workload": f"cavity {n}^3 Re {re:g} lid {lid}", "tau0": tau, "checks": [], "placement": {}}
-    trials = 8 if a.placements is None else a.placements
     for name, g in engines.items():
         g.Initialize()
-        if trials > 1 and name != "phased":  # the phased engine keeps its own layout
-            kept, ms_each = g.tune_placement(trials, steps=20)
-            out["placement"][name] = {"kept": kept, "glups_each": [
-                round(n ** 3 / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
-            log(f"[cavity] {name} placement: {out['placement'][name]}")
+        if name != "phased":  # the phased engine keeps its own layout
+            out["placement"][name] = place(g, a, n ** 3, f"cavity {name}")
     t = {k: 0.0 for k in engines}
     every = 1000
     for s in range(every, steps + 1, every):
@@ -147,14 +178,9 @@ def sphere(a):
                     sphere=sph) for dt in ("f64", "f32", "f16")}
     out = {"workload": f"sphere channel {nx}x{ny}x{nz} D {d:g} Re {re:g} u_in {u_in}", "tau0": tau, "drag": [],
            "placement": {}}
-    trials = 8 if a.placements is None else a.placements
     for dt, g in runs.items():
         g.autotune(None, steps=4)
-        if trials > 1:
-            kept, ms_each = g.tune_placement(trials, steps=10)
-            out["placement"][dt] = {"kept": kept, "glups_each": [
-                round(nx * ny * nz / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
-            log(f"[sphere] {dt} placement: {out['placement'][dt]}")
+        out["placement"][dt] = place(g, a, nx * ny * nz, f"sphere {dt}")
     every = 1000
     t_dev = {dt: 0.0 for dt in runs}
     for s in range(every, a.steps + 1, every):
@@ -186,11 +212,18 @@ def ensemble(a):
     e = Ensemble(doms, dtype="f32", storage="pull")
     e.Initialize()
     out["autotune"] = e.autotune(None, steps=4)
-    trials = 8 if a.placements is None else a.placements
-    if trials > 1:
-        kept, ms_each = e.tune_placement(trials, steps=6)
-        out["placement"] = {"trials": trials, "kept": kept,
-                            "glups_each": [round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
+    def rate(m):
+        return round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None
+    if a.placements is not None:
+        if a.placements > 1:
+            kept, ms_each = e.tune_placement(a.placements, steps=6)
+            out["placement"] = {"method": "separate allocations", "kept": kept,
+                                "glups_each": [rate(m) for m in ms_each]}
+    else:
+        kept, ms_each = e.tune_placement_window(slack_gib=64, step_gib=2, steps=6)
+        out["placement"] = {"method": "window", "slack_gib": 64, "step_gib": 2, "kept": kept,
+                            "glups_each": [rate(m) for m in ms_each]}
+    if out.get("placement"):
         log(f"[ensemble] placement: {out['placement']}")
     res = {}
     for mode in ("graph", "persistent", "per_domain"):
@@ -241,9 +274,9 @@ def timing(a):
     else:
         g.autotune(None, steps=8)
     out = {"workload": f"TGV {'x'.join(map(str, shape))} Re 1600 {a.dtype} {a.storage}", "placement": None}
-    if (a.placements or 1) > 1:
-        kept, ms_each = g.tune_placement(a.placements, steps=10)
-        out["placement"] = {"kept": kept, "ms_per_step_each": ms_each}
+    if a.placements is None:
+        a.placements = 1  # timing keeps the first allocation unless asked
+    out["placement"] = place(g, a, shape[0] * shape[1] * shape[2], "timing")
     g.step(4, moments_every=0)
     ms = g.step_timed(a.steps, moments_every=0)
     info = g.info()
@@ -266,18 +299,12 @@ def phased(a):
                                                             tau=0.05 * 2 * r / 100.0 / (1.0 / 3.0), u0=0.05,
                                                             case="sphere", engine="fused")}
     out = {"workload": "faithful mode, phased and fused engines", "runs": {}, "placement": {}}
-    trials = 8 if a.placements is None else a.placements
     for name, c in cases.items():
         g = LBM(*c["shape"], c["tau"], u0=c["u0"], case=c["case"], mode="faithful", engine=c["engine"])
         g.Initialize()
         if c["engine"] == "fused":  # launch shape and placement (the phased engine keeps its own layout)
             g.autotune(None, steps=8)
-            if trials > 1:
-                kept, ms_each = g.tune_placement(trials, steps=10)
-                cells = c["shape"][0] * c["shape"][1] * c["shape"][2]
-                out["placement"][name] = {"kept": kept, "glups_each": [
-                    round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None for m in ms_each]}
-                log(f"[phased] {name} placement: {out['placement'][name]}")
+            out["placement"][name] = place(g, a, c["shape"][0] * c["shape"][1] * c["shape"][2], f"phased {name}")
         g.Initialize()
         g.step(3)
         ms = g.step_timed(a.steps)
@@ -288,9 +315,36 @@ def phased(a):
     return out
 
 
+def placement(a):
+    n = a.n
+    u0, re = 0.05, 1600.0
+    tau = u0 * n / re / (1.0 / 3.0) / 2 / PI_R
+    cells = n ** 3
+
+    def rate(m):
+        return round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None
+    out = {"workload": f"TGV {n}^3 {a.storage}", "runs": {}}
+    for dt, launch, step in (("f64", (128, 1, 1, 1), 2), ("f32", (128, 1, 1, 1), 1), ("f16", (512, 1, 0, 1), 1)):
+        g = LBM(n, n, n, tau, u0=u0, dtype=dt, storage=a.storage)
+        g.set_launch(*launch)
+        g.Initialize()
+        for rep in range(2):
+            kept, ms = g.tune_placement_window(slack_gib=64, step_gib=step, steps=10)
+            r = [rate(m) for m in ms]
+            out["runs"][f"{dt} window {rep}"] = {"launch": launch, "step_gib": step, "kept": kept, "glups": r}
+            log(f"[placement] {dt} window {rep}: kept offset {kept} ({r[kept]}), min {min(r)}, max {max(r)}")
+        kept, ms = g.tune_placement(4, steps=10)
+        r = [rate(m) for m in ms]
+        out["runs"][f"{dt} allocations"] = {"launch": launch, "kept": kept, "glups": r}
+        log(f"[placement] {dt} separate allocations: {r}")
+        g.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["precision", "cavity", "sphere", "ensemble", "timing", "phased"])
+    ap.add_argument("workload", choices=["precision", "cavity", "sphere", "ensemble", "timing", "phased",
+                                         "placement"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--domains", type=int, default=256)
@@ -298,16 +352,17 @@ def main(argv=None):
     ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
     ap.add_argument("--launch", default=None, help="block,cells_per_thread,xcd_order,nt_loads")
     ap.add_argument("--placements", type=int, default=None,
-                    help="lattice allocations to try (default 1 for timing, 8 for the others)")
+                    help="separate lattice allocations to try instead of the window search (1: the first "
+                         "allocation; timing defaults to 1)")
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),
-                "timing": (512, 20), "phased": (256, 20)}
+                "timing": (512, 20), "phased": (256, 20), "placement": (512, 10)}
     dn, ds = defaults[a.workload]
     a.n = a.n or dn
     a.steps = a.steps or ds
     fn = {"precision": precision, "cavity": cavity, "sphere": sphere, "ensemble": ensemble, "timing": timing,
-          "phased": phased}[a.workload]
+          "phased": phased, "placement": placement}[a.workload]
     out = fn(a)
     out["copy_GBps"] = copy_bandwidth(1 << 30, reps=5)[0]
     print(json.dumps(out), flush=True)

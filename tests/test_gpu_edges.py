@@ -137,6 +137,47 @@ def test_placement_search_keeps_results(storage):
         ph.tune_placement(2)
 
 
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_placement_window_keeps_results(storage):
+    """The window placement search slides the lattice through one allocation
+    of lattice + slack, keeps the fastest offset and holds the window: the
+    run that follows is the oracle's, bit for bit, also after a second window
+    search and after a separate-allocation search that starts from a window;
+    the handle's device memory grows by the slack only while it holds a
+    window; bad arguments come back as errors."""
+    from imexlb_amd import LBM, ImexlbError
+    n = (64, 24, 20)
+    tau = tgv_tau(n[0])
+    g = LBM(*n, tau, u0=0.05, storage=storage)
+    g.Initialize()
+    base = g.info()["device_bytes"]
+    slack = 8 << 20
+
+    def oracle_after_7():
+        # assert_bitexact collides the oracle in place: a fresh one per check
+        o = Oracle(TGV, *n, tau0=tau, u0=0.05)
+        o.Initialize()
+        o.run(7)
+        return o
+    for rep in range(2):
+        print(f"[window] {storage} search {rep}", flush=True)
+        kept, ms = g.tune_placement_window(slack_gib=8 / 1024, step_gib=2 / 1024, steps=4)
+        assert len(ms) == 5 and 0 <= kept < 5 and all(m > 0 for m in ms) and ms[kept] == min(ms)
+        assert g.info()["steps"] == 0
+        assert g.info()["device_bytes"] == base + slack
+        g.step(7)
+        assert_bitexact(oracle_after_7(), g)
+    kept, ms = g.tune_placement(3, steps=4)
+    g.Initialize()
+    g.step(7)
+    assert_bitexact(oracle_after_7(), g)
+    assert g.info()["device_bytes"] == (base + slack if kept == 0 else base)
+    for slack_gib, step_gib in ((8 / 1024, 3 / 1024), (8 / 1024, 1 / 1024)):
+        with pytest.raises(ImexlbError):
+            g.tune_placement_window(slack_gib=slack_gib, step_gib=step_gib)
+    g.close()
+
+
 @pytest.mark.parametrize("dtype", ["f32", "f16"])
 @pytest.mark.parametrize("storage", ["pull", "aa"])
 def test_reduced_precision_bits_do_not_depend_on_launch_shape(dtype, storage):

@@ -114,16 +114,32 @@ def test_ensemble_placement_keeps_results(storage):
     e.Initialize()
     e.step(9, mode="graph")
     ref = [e.moments(k) for k in range(len(doms))]
+    bytes0 = e.info()["device_bytes"]
     kept, ms = e.tune_placement(trials=3, steps=4)
     print(f"[ensemble placement] {storage}: kept {kept} of ms/step {ms}", flush=True)
     assert 0 <= kept < 3 and ms[kept] > 0 and ms[kept] == min(m for m in ms if m > 0)
     assert e.info()["steps"] == 0
-    for mode in ("graph", "per_domain"):
-        e.Initialize()
-        e.step(9, mode=mode)
-        for k in range(len(doms)):
-            for a, b in zip(e.moments(k), ref[k]):
-                assert np.array_equal(a, b), (storage, mode, k)
+
+    def check():
+        for mode in ("graph", "per_domain"):
+            e.Initialize()
+            e.step(9, mode=mode)
+            for k in range(len(doms)):
+                for a, b in zip(e.moments(k), ref[k]):
+                    assert np.array_equal(a, b), (storage, mode, k)
+    check()
+    # the window search: the set slides through one allocation of its size
+    # plus 6 MiB in 2 MiB steps, twice, then a separate-allocation search
+    # starts from the window
+    for rep in range(2):
+        kept, ms = e.tune_placement_window(slack_gib=6 / 1024, step_gib=2 / 1024, steps=4)
+        print(f"[ensemble window] {storage} {rep}: kept {kept} of ms/step {ms}", flush=True)
+        assert len(ms) == 4 and 0 <= kept < 4 and ms[kept] == min(ms)
+        assert e.info()["device_bytes"] == bytes0 + (6 << 20)
+        check()
+    kept, ms = e.tune_placement(trials=2, steps=4)
+    assert e.info()["device_bytes"] == (bytes0 + (6 << 20) if kept == 0 else bytes0)
+    check()
 
 
 # ---------------------------------------------------------------- the seeded 256-domain ensemble
