@@ -252,8 +252,9 @@ def main():
     ap.add_argument("--placement", default="window", choices=["window", "allocations", "none"],
                     help="lattice placement search: offsets in one window of device memory chunks (default), "
                          "separate allocations, or the first allocation")
-    ap.add_argument("--window-slack-gib", type=float, default=64.0,
-                    help="window beyond the lattice for --placement window (held during the search only)")
+    ap.add_argument("--window-slack-gib", type=float, default=96.0,
+                    help="window beyond the lattice for --placement window; the fast offsets recur irregularly, "
+                         "about 30-60 GiB apart (the handle keeps the window)")
     ap.add_argument("--placements", type=int, default=None,
                     help="time this many separate allocations instead (1: the first allocation only)")
     ap.add_argument("--placement-budget-gb", type=float, default=128.0,

@@ -41,18 +41,18 @@ __device__ __forceinline__ void tile_run(const FusedParams &P, uint32_t first, b
     CellCtx k[CPT];
     R f[CPT][Q];
     const S *at[CPT][Q];
-    // every member's populations span at most 4 GiB (imexlb_ensemble_create
-    // checks): 32-bit offsets inside a population
+    // every member holds at most 2^28 cells per population
+    // (imexlb_ensemble_create checks): 32-bit offsets, grouped populations
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         cell_setup<BC>(P, first + (uint32_t)r * blockDim.x + threadIdx.x, k[r]);
-        if (k[r].live) cell_gather<S, BC, PAT, NTL, true>(P, k[r], f[r], at[r]);
+        if (k[r].live) cell_gather<S, BC, PAT, NTL, AM_GROUP>(P, k[r], f[r], at[r]);
     }
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         if (!k[r].live) continue;
         cell_collide<S, BC>(P, k[r], f[r], wm);
-        cell_scatter<S, BC, PAT, true, PAT == PAT_AA_ODD && CPT == 1>(P, k[r], f[r], at[r]);
+        cell_scatter<S, BC, PAT, AM_GROUP, PAT == PAT_AA_ODD && CPT == 1>(P, k[r], f[r], at[r]);
     }
 }
 
@@ -369,8 +369,9 @@ int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int stor
     for (int d = 0; d < n; ++d) {
         const uint64_t nc = (uint64_t)doms[d].nx * doms[d].ny * doms[d].nz, per = 256 / elem;
         pstride[d] = (nc + per - 1) / per * per + (uint64_t)IMEXLB_DEFAULT_PAD / elem;
-        // the member kernels address a population with 32-bit byte offsets
-        if (pstride[d] * elem > (1ull << 32)) return bail(IMEXLB_EINVAL);
+        // the member kernels address a group of populations (16 bytes of
+        // stride per cell, PopGroup) with 32-bit byte offsets
+        if (pstride[d] * 16 > (1ull << 32)) return bail(IMEXLB_EINVAL);
         for (int k = 0; k < nl; ++k) {
             e->lat_off[2 * d + k] = e->pool_bytes;
             e->pool_bytes += ((size_t)Q * pstride[d] * elem + 4095) / 4096 * 4096;

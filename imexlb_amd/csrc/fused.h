@@ -58,28 +58,52 @@ __device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t
     return ld_ptr<S, NT>(lat + at, cls);
 }
 
-// Cell c of population i.  W32: through population i's array as a uniform
-// base plus the cell's byte offset as a 32-bit value -- base + zext(offset)
-// is the global saddr address form, one VGPR per address instead of a 64-bit
-// pair, which keeps the AA odd step (27 neighbour addresses loaded and
-// stored) at the register count of the other steps.  Valid while one
-// population spans at most 4 GiB, which the launch checks.
-template <typename S, bool W32>
+// How the kernels address a population (the launch picks the mode from the
+// population stride; every mode gives the same results):
+// - AM_WIDE: 64-bit element offsets.
+// - AM_W32: population i's array as a uniform base plus the cell's byte
+//   offset as a 32-bit value.  base + zext(offset) is the global saddr form,
+//   one VGPR per address instead of a 64-bit pair, which keeps the AA odd step
+//   (27 neighbour addresses loaded and stored) at the register count of the
+//   other steps.  Needs one population within 4 GiB.
+// - AM_GROUP: as AM_W32, and where every population is addressed at the same
+//   cell (the AA even step, the pull step's stores) the populations are taken
+//   in groups of POP_GROUP (16 bytes of population stride per cell): one
+//   uniform base per group, and the population's place in the group added to
+//   the 32-bit offset.  With a base per population there the compiler forms
+//   lattice + offset once in VGPRs and adds each population's 64-bit offset
+//   in VALU; with the groups the step keeps a few scalar bases (FP16 even
+//   step: 64 instead of 103 VGPRs).  Needs a group within 4 GiB: at most 2^28
+//   cells per population.
+enum AddrMode : int { AM_WIDE = 0, AM_W32 = 1, AM_GROUP = 2 };
+
+template <typename S>
+struct PopGroup { static constexpr int n = 16 / (int)sizeof(S); };
+
+template <typename S, int AM, bool SAME_CELL = false>
 __device__ __forceinline__ S *pop_ptr(S *lat, uint64_t pstride, int i, uint32_t c)
 {
-    if constexpr (W32) {
-        auto *base = (typename std::conditional<std::is_const<S>::value, const char, char>::type *)(
-            lat + (uint64_t)i * pstride);
-        return (S *)(base + (uint32_t)(c * (uint32_t)sizeof(S)));
+    using B = typename std::conditional<std::is_const<S>::value, const char, char>::type;
+    if constexpr (AM == AM_GROUP && SAME_CELL) {
+        constexpr int G = PopGroup<S>::n;
+        // the group's base through readfirstlane: a uniform value the
+        // compiler cannot fold back into the per-lane address arithmetic
+        const uint64_t a = (uint64_t)(lat + (uint64_t)(i / G * G) * pstride);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+        const uint32_t o = (uint32_t)(i % G) * (uint32_t)(pstride * sizeof(S)) + c * (uint32_t)sizeof(S);
+        return (S *)((B *)(((uint64_t)hi << 32) | lo) + o);
+    } else if constexpr (AM != AM_WIDE) {
+        return (S *)((B *)(lat + (uint64_t)i * pstride) + (uint32_t)(c * (uint32_t)sizeof(S)));
     } else {
         return lat + (uint64_t)i * pstride + c;
     }
 }
 
-template <typename S, bool NT = false, bool W32 = false>
+template <typename S, bool NT = false, int AM = AM_WIDE>
 __device__ __forceinline__ typename RealOf<S>::type ld_pop(const S *lat, uint64_t pstride, int i, uint32_t c)
 {
-    return ld_ptr<S, NT>(pop_ptr<const S, W32>(lat, pstride, i, c), WCLASS[i]);
+    return ld_ptr<S, NT>(pop_ptr<const S, AM>(lat, pstride, i, c), WCLASS[i]);
 }
 
 // NTL (non-temporal loads) variant of the gather: lines that no other
@@ -106,10 +130,10 @@ __device__ __forceinline__ void st_ptr(S *ptr, int cls, typename RealOf<S>::type
     else *p = x;
 }
 
-template <typename S, bool NT = true, bool W32 = false>
+template <typename S, bool NT = true, int AM = AM_WIDE, bool SAME_CELL = false>
 __device__ __forceinline__ void st_pop(S *lat, uint64_t pstride, int i, uint32_t c, typename RealOf<S>::type v)
 {
-    st_ptr<S, NT>(pop_ptr<S, W32>(lat, pstride, i, c), WCLASS[i], v);
+    st_ptr<S, NT>(pop_ptr<S, AM, SAME_CELL>(lat, pstride, i, c), WCLASS[i], v);
 }
 
 // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
@@ -375,9 +399,10 @@ __device__ __forceinline__ uint32_t src_off(const CellCtx &k, int i)
 // stored where direction bb(i) was loaded from (even step: slot bb(i) of the
 // cell; odd step: slot i of x + e_i, which the load of bb(i) read from x -
 // e_bb(i)).  The gather hands its addresses to the scatter through `at`, so
-// the step keeps one set of addresses -- a scalar base and a 32-bit vector
-// offset each with W32 -- instead of forming a second set for the stores.
-template <typename S, int BC, int PAT, bool NTL = false, bool W32 = false>
+// the odd step keeps one set of addresses -- a scalar base and a 32-bit
+// vector offset each with AM_W32 -- instead of forming a second set for the
+// stores.
+template <typename S, int BC, int PAT, bool NTL = false, int AM = AM_WIDE>
 __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx &k,
                                             typename RealOf<S>::type (&f)[Q], const S *(&at)[Q])
 {
@@ -389,7 +414,7 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
     if constexpr (PAT == PAT_AA_EVEN) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            at[i] = pop_ptr<const S, W32>(src, ps, i, c);
+            at[i] = pop_ptr<const S, AM, true>(src, ps, i, c);
             f[i] = ld_ptr<S, NTL>(at[i], WCLASS[i]);
             if constexpr (BC == BC_CAVITY) {
                 // the odd step stored the bounced population raw; the lid
@@ -407,7 +432,7 @@ __device__ __forceinline__ void cell_gather(const FusedParams &P, const CellCtx 
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             if constexpr (BC == BC_PERIODIC) {
-                at[i] = pop_ptr<const S, W32>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
+                at[i] = pop_ptr<const S, AM>(src, ps, PAT == PAT_PULL ? i : BB[i], src_off(k, i));
                 if (NTL && EX[i] == 0) f[i] = ld_ptr<S, true>(at[i], WCLASS[i]);
                 else f[i] = ld_ptr<S, false>(at[i], WCLASS[i]);
             } else if constexpr (BC == BC_CAVITY) {
@@ -547,7 +572,7 @@ __device__ __forceinline__ void cell_collide(const FusedParams &P, const CellCtx
 // cell_gather); otherwise the store addresses are formed again from the
 // cell, which is cheaper in registers when the addresses of two cells per
 // thread, or the even step's in-place addresses, would be held.
-template <typename S, int BC, int PAT, bool W32 = false, bool REUSE = false>
+template <typename S, int BC, int PAT, int AM = AM_WIDE, bool REUSE = false>
 __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx &k,
                                              const typename RealOf<S>::type (&f)[Q], const S *const (&at)[Q])
 {
@@ -556,10 +581,10 @@ __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx
     const uint32_t c = k.c;
     if constexpr (PAT == PAT_PULL) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) st_pop<S, true, W32>(dst, ps, i, c, f[i]);
+        for (int i = 0; i < Q; ++i) st_pop<S, true, AM, true>(dst, ps, i, c, f[i]);
     } else if constexpr (PAT == PAT_AA_EVEN) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) st_pop<S, true, W32>(dst, ps, BB[i], c, f[i]);
+        for (int i = 0; i < Q; ++i) st_pop<S, true, AM, true>(dst, ps, BB[i], c, f[i]);
     } else {
         // to the natural slot of x + e_i, or bounced back into slot bb(i) at
         // x: in both cases where direction bb(i) was loaded from.  Shifted
@@ -573,8 +598,8 @@ __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx
             S *p;
             if constexpr (REUSE) p = (S *)at[BB[i]];
             else if constexpr (BC == BC_CAVITY)
-                p = bounced ? pop_ptr<S, W32>(dst, ps, BB[i], c) : pop_ptr<S, W32>(dst, ps, i, src_off(k, BB[i]));
-            else p = pop_ptr<S, W32>(dst, ps, i, src_off(k, BB[i]));
+                p = bounced ? pop_ptr<S, AM>(dst, ps, BB[i], c) : pop_ptr<S, AM>(dst, ps, i, src_off(k, BB[i]));
+            else p = pop_ptr<S, AM>(dst, ps, i, src_off(k, BB[i]));
             if (plain) st_ptr<S, false>(p, WCLASS[i], f[i]);
             else st_ptr<S>(p, WCLASS[i], f[i]);
         }
@@ -583,8 +608,8 @@ __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx
 
 // CPT cells per thread, blockDim.x apart; the loads of all CPT cells are
 // issued before any of them is collided, to keep more bytes in flight.
-// W32: 32-bit offsets inside each population (see pop_ptr).
-template <typename S, int BC, int PAT, int CPT, bool NTL, bool W32>
+// AM: the address mode (AddrMode).
+template <typename S, int BC, int PAT, int CPT, bool NTL, int AM>
 __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
 {
     using R = typename RealOf<S>::type;
@@ -597,21 +622,21 @@ __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         cell_setup<BC>(P, base + (uint32_t)r * blockDim.x, k[r]);
-        if (k[r].live) cell_gather<S, BC, PAT, NTL, W32>(P, k[r], f[r], at[r]);
+        if (k[r].live) cell_gather<S, BC, PAT, NTL, AM>(P, k[r], f[r], at[r]);
     }
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         if (!k[r].live) continue;
         cell_collide<S, BC>(P, k[r], f[r], P.mom_p != nullptr);
-        cell_scatter<S, BC, PAT, W32, PAT == PAT_AA_ODD && CPT == 1>(P, k[r], f[r], at[r]);
+        cell_scatter<S, BC, PAT, AM, PAT == PAT_AA_ODD && CPT == 1>(P, k[r], f[r], at[r]);
     }
 }
 
-template <typename S, int CPT, bool NTL, bool W32>
+template <typename S, int CPT, bool NTL, int AM>
 hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
     dim3 grid(P.nblocks), blk(block);
-#define IMX_L(B, PT) hipLaunchKernelGGL((k_fused<S, B, PT, CPT, NTL, W32>), grid, blk, 0, st, P)
+#define IMX_L(B, PT) hipLaunchKernelGGL((k_fused<S, B, PT, CPT, NTL, AM>), grid, blk, 0, st, P)
     if (bc == BC_PERIODIC) {
         if (pattern == PAT_PULL) IMX_L(BC_PERIODIC, PAT_PULL);
         else if (pattern == PAT_AA_EVEN) IMX_L(BC_PERIODIC, PAT_AA_EVEN);
@@ -637,25 +662,37 @@ hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block
 }
 
 // Does every element offset inside one population fit the 32-bit byte
-// offsets of the W32 kernels?
+// offsets of AM_W32?  Does a group of PopGroup populations fit those of
+// AM_GROUP?
 template <typename S>
 inline bool fits_w32(uint64_t pstride)
 {
     return pstride * sizeof(S) <= (uint64_t(1) << 32);
 }
+template <typename S>
+inline bool fits_group(uint64_t pstride)
+{
+    return pstride * sizeof(S) * PopGroup<S>::n <= (uint64_t(1) << 32);
+}
 
 template <typename S>
 hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
-    // lattices with populations beyond 4 GiB: 64-bit addresses, one cell
-    // per thread, plain loads (the same results)
-    if (!fits_w32<S>(P.pstride)) return launch_fused_cpt<S, 1, false, false>(bc, pattern, P, block, st);
-    if (P.ntl) {
-        if (P.cpt == 2) return launch_fused_cpt<S, 2, true, true>(bc, pattern, P, block, st);
-        return launch_fused_cpt<S, 1, true, true>(bc, pattern, P, block, st);
+    if (fits_group<S>(P.pstride)) {
+        if (P.ntl) {
+            if (P.cpt == 2) return launch_fused_cpt<S, 2, true, AM_GROUP>(bc, pattern, P, block, st);
+            return launch_fused_cpt<S, 1, true, AM_GROUP>(bc, pattern, P, block, st);
+        }
+        if (P.cpt == 2) return launch_fused_cpt<S, 2, false, AM_GROUP>(bc, pattern, P, block, st);
+        return launch_fused_cpt<S, 1, false, AM_GROUP>(bc, pattern, P, block, st);
     }
-    if (P.cpt == 2) return launch_fused_cpt<S, 2, false, true>(bc, pattern, P, block, st);
-    return launch_fused_cpt<S, 1, false, true>(bc, pattern, P, block, st);
+    // larger lattices: one cell per thread (the same results); beyond 4 GiB
+    // per population 64-bit addresses and plain loads
+    if (fits_w32<S>(P.pstride)) {
+        if (P.ntl) return launch_fused_cpt<S, 1, true, AM_W32>(bc, pattern, P, block, st);
+        return launch_fused_cpt<S, 1, false, AM_W32>(bc, pattern, P, block, st);
+    }
+    return launch_fused_cpt<S, 1, false, AM_WIDE>(bc, pattern, P, block, st);
 }
 
 }  // namespace imx
