@@ -8,7 +8,8 @@ line.
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
     python -m imexlb_amd.runs phased    [--n 256] [--steps 20]
-    python -m imexlb_amd.runs placement [--n 512] [--storage aa]
+    python -m imexlb_amd.runs placement [--n 512] [--storage aa] [--dtypes f64,f32,f16] [--windows 2]
+                                        [--allocations 4]
 
 precision: TGV 512^3, U0 0.05, Re 1600, AA storage in FP64 / FP32 / FP16
     (shifted); kinetic energy against the analytic early-time decay
@@ -30,9 +31,10 @@ timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, a fix
     shape (or the autotuner's), `steps` timed steps after 4 warm-up steps;
     GLUPS and algorithmic GB/s.  The short run to profile one kernel with.
 placement: TGV n^3 in each storage type at a fixed launch shape: the step
-    rate with the lattice at every offset of a window search (twice, in two
-    separate windows: 64 GiB in 2 GiB steps for FP64, 1 GiB steps for FP32 and
-    FP16), then over four separate allocations.
+    rate with the lattice at every offset of a window search (--windows of
+    them, 2 by default, in separate windows: 64 GiB in 2 GiB steps for FP64,
+    1 GiB steps for FP32 and FP16), then over --allocations separate
+    allocations (4).
 phased: the reference's own boundary code (faithful mode) on the phased
     engine's multi-step path: the cavity at n^3 (lid 0.1, tau0 0.096) and the
     sphere input of sphere/input.in (512 x 128 x 128, Re 100, u0 0.05); and the
@@ -325,18 +327,21 @@ def placement(a):
         return round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None
     out = {"workload": f"TGV {n}^3 {a.storage}", "runs": {}}
     for dt, launch, step in (("f64", (128, 1, 1, 1), 2), ("f32", (128, 1, 1, 1), 1), ("f16", (512, 1, 0, 1), 1)):
+        if dt not in a.dtypes.split(","):
+            continue
         g = LBM(n, n, n, tau, u0=u0, dtype=dt, storage=a.storage)
         g.set_launch(*launch)
         g.Initialize()
-        for rep in range(2):
+        for rep in range(a.windows):
             kept, ms = g.tune_placement_window(slack_gib=64, step_gib=step, steps=10)
             r = [rate(m) for m in ms]
             out["runs"][f"{dt} window {rep}"] = {"launch": launch, "step_gib": step, "kept": kept, "glups": r}
             log(f"[placement] {dt} window {rep}: kept offset {kept} ({r[kept]}), min {min(r)}, max {max(r)}")
-        kept, ms = g.tune_placement(4, steps=10)
-        r = [rate(m) for m in ms]
-        out["runs"][f"{dt} allocations"] = {"launch": launch, "kept": kept, "glups": r}
-        log(f"[placement] {dt} separate allocations: {r}")
+        if a.allocations > 1:
+            kept, ms = g.tune_placement(a.allocations, steps=10)
+            r = [rate(m) for m in ms]
+            out["runs"][f"{dt} allocations"] = {"launch": launch, "kept": kept, "glups": r}
+            log(f"[placement] {dt} separate allocations: {r}")
         g.close()
     return out
 
@@ -355,6 +360,10 @@ def main(argv=None):
                     help="separate lattice allocations to try instead of the window search (1: the first "
                          "allocation; timing defaults to 1)")
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
+    ap.add_argument("--dtypes", default="f64,f32,f16", help="placement: storage types to search")
+    ap.add_argument("--windows", type=int, default=2, help="placement: window searches per storage type")
+    ap.add_argument("--allocations", type=int, default=4,
+                    help="placement: separate allocations timed after the windows (1: none)")
     a = ap.parse_args(argv)
     defaults = {"precision": (512, 5000), "cavity": (128, 20000), "sphere": (0, 50000), "ensemble": (0, 2000),
                 "timing": (512, 20), "phased": (256, 20), "placement": (512, 10)}

@@ -273,6 +273,8 @@ typedef struct imexlb_ensemble imexlb_ensemble;
 /* n domains drawn from seed: alternating TGV / cavity, edge in
  * {32,48,64,80,96}, Re in [50,1000], lid speed in [0.05,0.1], TGV U0 0.05. */
 int imexlb_ensemble_generate(uint64_t seed, int n, imexlb_domain *out);
+/* All members' lattices share one device allocation.  IMEXLB_EINVAL for a
+ * member of more than 2^28 cells (the member kernels' 32-bit offsets). */
 int imexlb_ensemble_create(const imexlb_domain *doms, int n, int dtype, int storage, int device,
                            imexlb_ensemble **out);
 int imexlb_ensemble_destroy(imexlb_ensemble *e);
