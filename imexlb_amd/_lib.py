@@ -34,6 +34,7 @@ class Config(ctypes.Structure):
         ("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
         ("xcd_remap", ctypes.c_int32), ("device", ctypes.c_int32),
         ("stride_pad", ctypes.c_int32), ("nt_loads", ctypes.c_int32),
+        ("addr_mode", ctypes.c_int32),
     ]
 
 
@@ -41,7 +42,8 @@ class Info(ctypes.Structure):
     _fields_ = [("cells", ctypes.c_int64), ("steps", ctypes.c_int64),
                 ("bytes_per_lup", ctypes.c_double), ("device_bytes", ctypes.c_int64),
                 ("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
-                ("xcd_remap", ctypes.c_int32), ("nt_loads", ctypes.c_int32)]
+                ("xcd_remap", ctypes.c_int32), ("nt_loads", ctypes.c_int32),
+                ("addr_mode", ctypes.c_int32)]
 
 
 class SystemC(ctypes.Structure):

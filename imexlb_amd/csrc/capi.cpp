@@ -113,6 +113,7 @@ int imexlb_config_init(imexlb_config *c)
     c->xcd_remap = -1;
     c->stride_pad = -1;
     c->nt_loads = -1;
+    c->addr_mode = -1;
     return IMEXLB_OK;
 }
 
@@ -160,6 +161,7 @@ static int validate(const imexlb_config *c)
         return fail(nullptr, IMEXLB_EINVAL, "stride_pad %d must be -1 or a multiple of 256 below 16 MiB", c->stride_pad);
     if (c->cells_per_thread < 0 || c->cells_per_thread > 2)
         return fail(nullptr, IMEXLB_EINVAL, "cells_per_thread %d", c->cells_per_thread);
+    if (c->addr_mode < -1 || c->addr_mode > 2) return fail(nullptr, IMEXLB_EINVAL, "addr_mode %d", c->addr_mode);
     if (c->engine == IMEXLB_ENGINE_FUSED) {
         if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id != IMEXLB_CASE_TGV &&
             (c->dtype != IMEXLB_F64 || c->storage != IMEXLB_PULL))
@@ -204,6 +206,7 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.xcd_remap = h->xcd;
     P.cpt = h->cpt;
     P.ntl = h->ntl;
+    P.am_max = c.addr_mode < 0 ? 2 : c.addr_mode;
     P.divx.init(c.nx);
     P.divy.init(c.ny);
     P.ring = h->ring;
@@ -931,6 +934,7 @@ int imexlb_get_info(imexlb_handle *h, imexlb_info *info)
     info->cells_per_thread = h->cpt;
     info->xcd_remap = h->xcd;
     info->nt_loads = h->ntl;
+    info->addr_mode = h->ph ? -1 : fused_addr_mode(h->elem, h->pstride, c.addr_mode < 0 ? 2 : c.addr_mode);
     return IMEXLB_OK;
 }
 

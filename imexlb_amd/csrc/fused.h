@@ -661,24 +661,12 @@ hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block
     return hipGetLastError();
 }
 
-// Does every element offset inside one population fit the 32-bit byte
-// offsets of AM_W32?  Does a group of PopGroup populations fit those of
-// AM_GROUP?
-template <typename S>
-inline bool fits_w32(uint64_t pstride)
-{
-    return pstride * sizeof(S) <= (uint64_t(1) << 32);
-}
-template <typename S>
-inline bool fits_group(uint64_t pstride)
-{
-    return pstride * sizeof(S) * PopGroup<S>::n <= (uint64_t(1) << 32);
-}
-
 template <typename S>
 hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
-    if (fits_group<S>(P.pstride)) {
+    static_assert(16 / sizeof(S) == (size_t)PopGroup<S>::n, "group of 16 bytes of stride per cell");
+    const int am = fused_addr_mode(sizeof(S), P.pstride, P.am_max);
+    if (am == AM_GROUP) {
         if (P.ntl) {
             if (P.cpt == 2) return launch_fused_cpt<S, 2, true, AM_GROUP>(bc, pattern, P, block, st);
             return launch_fused_cpt<S, 1, true, AM_GROUP>(bc, pattern, P, block, st);
@@ -686,13 +674,17 @@ hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, 
         if (P.cpt == 2) return launch_fused_cpt<S, 2, false, AM_GROUP>(bc, pattern, P, block, st);
         return launch_fused_cpt<S, 1, false, AM_GROUP>(bc, pattern, P, block, st);
     }
-    // larger lattices: one cell per thread (the same results); beyond 4 GiB
-    // per population 64-bit addresses and plain loads
-    if (fits_w32<S>(P.pstride)) {
-        if (P.ntl) return launch_fused_cpt<S, 1, true, AM_W32>(bc, pattern, P, block, st);
-        return launch_fused_cpt<S, 1, false, AM_W32>(bc, pattern, P, block, st);
+    // larger lattices: one cell per thread (the same results), so the grid
+    // covers block cells per workgroup whatever cells per thread was asked
+    FusedParams P1 = P;
+    P1.cpt = 1;
+    P1.nblocks = (P.ncell + (uint32_t)block - 1) / (uint32_t)block;
+    if (am == AM_W32) {
+        if (P.ntl) return launch_fused_cpt<S, 1, true, AM_W32>(bc, pattern, P1, block, st);
+        return launch_fused_cpt<S, 1, false, AM_W32>(bc, pattern, P1, block, st);
     }
-    return launch_fused_cpt<S, 1, false, AM_WIDE>(bc, pattern, P, block, st);
+    // beyond 4 GiB per population: 64-bit addresses, plain loads
+    return launch_fused_cpt<S, 1, false, AM_WIDE>(bc, pattern, P1, block, st);
 }
 
 }  // namespace imx

@@ -59,6 +59,18 @@ struct FastDiv {
 };
 
 // One fused stream + moments + collide launch over a single domain.
+// The address mode the fused kernels run with (fused.h AddrMode): 2 (a
+// uniform base per group of 16 / elem populations, 32-bit offsets) while such
+// a group spans at most 4 GiB, else 1 (a uniform base per population) while a
+// population does, else 0 (64-bit offsets); capped at am_max.
+inline int fused_addr_mode(uint64_t elem, uint64_t pstride, int am_max)
+{
+    const uint64_t lim = uint64_t(1) << 32;
+    if (am_max >= 2 && pstride * 16 <= lim) return 2;
+    if (am_max >= 1 && pstride * elem <= lim) return 1;
+    return 0;
+}
+
 struct FusedParams {
     int nx, ny, nz;
     uint32_t ncell;          // nx * ny * nz
@@ -79,6 +91,7 @@ struct FusedParams {
     int xcd_remap;           // 1: give each XCD one contiguous run of tiles
     int cpt;                 // cells per thread (1 or 2), blockDim apart
     int ntl;                 // 1: non-temporal loads of unshared lines
+    int am_max;              // widest-saving address mode allowed (AddrMode in fused.h)
     int bc;                  // boundary kind (read by the ensemble kernel)
     FastDiv divx, divy;
     // BC_FAITHFUL: the first ghost layer, population-major (q * ring_n + r).

@@ -37,7 +37,7 @@ class LBM:
     def __init__(self, sx, sy, sz, tau, rho0=1.0, u0=0.05, case="tgv", mode="faithful",
                  engine="fused", dtype="f64", storage="pull", lid_u=0.1, sphere=None,
                  seed=5374857, block=0, cells_per_thread=0, xcd_remap=-1, device=0, stride_pad=-1,
-                 nt_loads=-1):
+                 nt_loads=-1, addr_mode=-1):
         c = L.Config()
         check(lib().imexlb_config_init(ctypes.byref(c)))
         c.case_id = _code(CASES, case)
@@ -54,6 +54,7 @@ class LBM:
             int(xcd_remap), int(device)
         c.stride_pad = int(stride_pad)
         c.nt_loads = int(nt_loads)
+        c.addr_mode = int(addr_mode)
         self.cfg = c
         self.shape = (c.nz, c.ny, c.nx)
         h = ctypes.c_void_p()

@@ -4,7 +4,7 @@ line.
     python -m imexlb_amd.runs precision [--n 512] [--steps 5000]
     python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000] [--placements N]
     python -m imexlb_amd.runs sphere    [--steps 50000] [--placements N]
-    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--placements N]
+    python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--storage aa|pull] [--placements N]
     python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
     python -m imexlb_amd.runs phased    [--n 256] [--steps 20]
@@ -20,8 +20,9 @@ sphere: 512x256x256 channel, sphere D = 40 at (128, 128, 128), inlet 0.05,
     Re 200; FP64 / FP32 / FP16 pull; drag proxy every 1000 steps and the final
     velocity field compared with FP64.
 ensemble: 256 domains from seed 20261202 (TGV / cavity, edges 32..96,
-    Re 50..1000, lid 0.05..0.1), FP32 shifted storage; persistent launch with
-    graph replay vs persistent launches vs one launch per domain; per-domain
+    Re 50..1000, lid 0.05..0.1), FP32 shifted storage, pull (default) or AA;
+    window placement, then the launch autotuner; persistent launch with graph
+    replay vs persistent launches vs one launch per domain; per-domain
     velocity error against the same ensemble in FP64.
 Every run but timing places its lattices with the window search (the
 fastest offset of the lattice inside one allocation of lattice + 32-64 GiB);
@@ -210,10 +211,10 @@ def sphere(a):
 def ensemble(a):
     doms = Ensemble.generate(20261202, a.domains)
     cells = sum(d.nx * d.ny * d.nz for d in doms)
-    out = {"workload": f"{a.domains} domains seed 20261202, {cells} cells", "cells": cells}
-    e = Ensemble(doms, dtype="f32", storage="pull")
+    out = {"workload": f"{a.domains} domains seed 20261202, {cells} cells, {a.storage} storage", "cells": cells,
+           "storage": a.storage}
+    e = Ensemble(doms, dtype="f32", storage=a.storage)
     e.Initialize()
-    out["autotune"] = e.autotune(None, steps=4)
     def rate(m):
         return round(cells / (m * 1e-3) / 1e9, 2) if m > 0 else None
     if a.placements is not None:
@@ -227,6 +228,9 @@ def ensemble(a):
                             "glups_each": [rate(m) for m in ms_each]}
     if out.get("placement"):
         log(f"[ensemble] placement: {out['placement']}")
+    # launch shape on the kept placement
+    out["autotune"] = e.autotune(None, steps=4)
+    log(f"[ensemble] autotuned (block, cells/thread, grid): {out['autotune']}")
     res = {}
     for mode in ("graph", "persistent", "per_domain"):
         e.Initialize()
@@ -239,7 +243,7 @@ def ensemble(a):
     # per-domain error: same ensemble in FP64, same number of steps
     e.Initialize()
     e.step(a.steps, mode="graph")
-    ref = Ensemble(doms, dtype="f64", storage="pull")
+    ref = Ensemble(doms, dtype="f64", storage=a.storage)
     ref.Initialize()
     ref.step(a.steps, mode="graph")
     errs, fin64, fin32 = [], [], []
@@ -354,7 +358,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--domains", type=int, default=256)
     ap.add_argument("--dtype", default="f16", choices=["f64", "f32", "f16"])
-    ap.add_argument("--storage", default="aa", choices=["aa", "pull"])
+    ap.add_argument("--storage", default=None, choices=["aa", "pull"],
+                    help="population storage (default: pull for the ensemble, AA otherwise)")
     ap.add_argument("--launch", default=None, help="block,cells_per_thread,xcd_order,nt_loads")
     ap.add_argument("--placements", type=int, default=None,
                     help="separate lattice allocations to try instead of the window search (1: the first "
@@ -370,6 +375,10 @@ def main(argv=None):
     dn, ds = defaults[a.workload]
     a.n = a.n or dn
     a.steps = a.steps or ds
+    if a.storage is None:
+        # the ensemble's AA kernels run 20 % below its pull kernels (400 steps
+        # of the 256-domain draw: 21.4 against 26.9 GLUPS)
+        a.storage = "pull" if a.workload == "ensemble" else "aa"
     fn = {"precision": precision, "cavity": cavity, "sphere": sphere, "ensemble": ensemble, "timing": timing,
           "phased": phased, "placement": placement}[a.workload]
     out = fn(a)

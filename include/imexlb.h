@@ -85,6 +85,12 @@ typedef struct imexlb_config {
                                  streams off a common power-of-two alignment */
     int32_t nt_loads;         /* -1 = default, 0/1: non-temporal loads of the
                                  lines no other wavefront reads */
+    int32_t addr_mode;        /* fused kernels' addressing, the same results in
+                                 every mode: -1 = default (the most compact the
+                                 lattice allows), or at most 2 (a uniform base per
+                                 group of populations; up to 2^28 cells per
+                                 population), 1 (a base per population; up to
+                                 4 GiB per population), 0 (64-bit offsets) */
 } imexlb_config;
 
 typedef struct imexlb_handle imexlb_handle;
@@ -95,6 +101,7 @@ typedef struct imexlb_info {
     double bytes_per_lup;   /* algorithmic HBM bytes per cell update */
     int64_t device_bytes;   /* device memory held by the handle */
     int32_t block, cells_per_thread, xcd_remap, nt_loads;
+    int32_t addr_mode;      /* the fused kernels' address mode; -1 phased engine */
 } imexlb_info;
 
 /* Defaults: TGV, faithful, fused, FP64, pull, 64^3, tau0 = 0.0153, rho0 = 1,

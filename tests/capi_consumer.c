@@ -25,8 +25,12 @@ int main(int argc, char **argv)
     imexlb_config c;
     CHECK(imexlb_config_init(&c) == IMEXLB_OK);
     CHECK(c.case_id == IMEXLB_CASE_TGV && c.dtype == IMEXLB_F64);
-    c.nx = 1; /* extents below 2 are refused before any device call */
+    CHECK(c.addr_mode == -1 && c.nt_loads == -1);
     imexlb_handle *h = NULL;
+    c.addr_mode = 3; /* no such address mode: refused before any device call */
+    CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
+    c.addr_mode = -1;
+    c.nx = 1; /* extents below 2 are refused before any device call */
     CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
     CHECK(strlen(imexlb_last_error(NULL)) > 0);
     printf("ok create rejects: %s\n", imexlb_last_error(NULL));

@@ -243,3 +243,45 @@ def test_graph_replay_equals_single_launches(storage, dtype):
     b.step(33, 33)
     for x, y in zip(a.moments(), b.moments()):
         assert np.array_equal(x, y)
+
+
+ADDR_CASES = [
+    # case, mode, extents, tau0, u0, dtype, storage, sphere
+    ("tgv", "faithful", (67, 13, 11), None, 0.05, "f64", "pull", None),
+    ("tgv", "faithful", (67, 13, 11), None, 0.05, "f64", "aa", None),
+    ("tgv", "faithful", (67, 13, 11), None, 0.05, "f32", "aa", None),
+    ("tgv", "faithful", (67, 13, 11), None, 0.05, "f16", "pull", None),
+    ("tgv", "faithful", (67, 13, 11), None, 0.05, "f16", "aa", None),
+    ("cavity", "config", (24, 20, 18), 0.096, 0.0, "f64", "aa", None),
+    ("cavity", "config", (24, 20, 18), 0.096, 0.0, "f16", "pull", None),
+    ("sphere", "config", (30, 14, 16), 0.05, 0.05, "f64", "pull", (10.0, 7.0, 8.0, 3.5)),
+    ("sphere", "config", (30, 14, 16), 0.05, 0.05, "f16", "pull", (10.0, 7.0, 8.0, 3.5)),
+    ("cavity", "faithful", (24, 20, 18), 0.096, 0.0, "f64", "pull", None),
+    ("sphere", "faithful", (72, 12, 9), 0.08, 0.1, "f64", "pull", None),
+]
+
+
+@pytest.mark.parametrize("case,mode,n,tau,u0,dtype,storage,sphere", ADDR_CASES)
+def test_address_modes_give_identical_bits(case, mode, n, tau, u0, dtype, storage, sphere):
+    """The fused kernels' three address modes (grouped uniform bases, a base
+    per population, 64-bit offsets; the launch picks the most compact one the
+    lattice allows) compute the same addresses: forcing each on a small
+    lattice gives the same populations and moments, bit for bit."""
+    from imexlb_amd import LBM
+    print(f"[case] address modes {case} {mode} {n} {dtype} {storage}", flush=True)
+    tau = tgv_tau(n[0]) if tau is None else tau
+    out = []
+    for am in (2, 1, 0):
+        for shape in ((256, 1, 1, 1), (128, 2, 0, 0)):
+            g = LBM(*n, tau, u0=u0, case=case, mode=mode, dtype=dtype, storage=storage, sphere=sphere,
+                    addr_mode=am)
+            assert g.info()["addr_mode"] == am
+            g.set_launch(*shape)
+            g.Initialize()
+            g.step(7, 3)
+            out.append((am, shape, g.get_f(), g.moments()))
+            g.close()
+    for am, shape, f, m in out[1:]:
+        assert np.array_equal(out[0][2], f), f"populations, mode {am}, launch {shape}"
+        for name, a, b in zip("puvw", out[0][3], m):
+            assert np.array_equal(a, b), f"moment {name}, mode {am}, launch {shape}"
