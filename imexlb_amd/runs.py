@@ -376,8 +376,8 @@ def main(argv=None):
     a.n = a.n or dn
     a.steps = a.steps or ds
     if a.storage is None:
-        # the ensemble's AA kernels run 20 % below its pull kernels (400 steps
-        # of the 256-domain draw: 21.4 against 26.9 GLUPS)
+        # the ensemble runs pull a little faster than AA (400 steps of the
+        # 256-domain draw: 26.8-27.1 against 26.1-26.2 GLUPS)
         a.storage = "pull" if a.workload == "ensemble" else "aa"
     fn = {"precision": precision, "cavity": cavity, "sphere": sphere, "ensemble": ensemble, "timing": timing,
           "phased": phased, "placement": placement}[a.workload]
