@@ -71,8 +71,8 @@ __device__ __forceinline__ T read_const(const T *p)
 #endif
 }
 
-// Persistent over one boundary kind's run of the tile table: every
-// workgroup exits after its last tile.  One kernel per boundary kind keeps
+// Persistent over one boundary kind's run of the tile table (or one
+// workgroup per tile, grid 0): every workgroup exits after its last tile.  One kernel per boundary kind keeps
 // each at the registers of its own cell code (the periodic and cavity code in
 // one kernel held FP32 pull at 98 VGPRs and the AA odd step at 138, against
 // 68 and 108 for the periodic code alone).
@@ -80,7 +80,12 @@ template <typename S, int BC, int PAT, int CPT>
 __global__ void __launch_bounds__(512) k_ensemble(const EnsParams EP)
 {
     const bool wm = EP.write_moments != 0;
-    for (uint32_t t = EP.t0 + blockIdx.x; t < EP.ntiles; t += gridDim.x) {
+    const uint32_t count = EP.ntiles - EP.t0;
+    for (uint32_t v = blockIdx.x; v < count; v += gridDim.x) {
+        // XCD-aware order (v and blockIdx.x agree mod 8 when the grid is a
+        // multiple of 8, as the tuned grids are): each XCD walks one
+        // contiguous part of the run, so neighbouring tiles share its L2
+        const uint32_t t = EP.t0 + xcd_tile(v, count);
         const uint2 tl = read_const(EP.tiles + t);
         const FusedParams P = read_const(EP.doms + 2 * tl.x + (uint32_t)EP.parity);
         // non-temporal loads of unshared lines (measured faster on every
@@ -250,7 +255,8 @@ static int launch_persistent(imexlb_ensemble *e, int pattern, int parity, bool w
         if (bounds[r + 1] == bounds[r]) continue;
         EP.t0 = bounds[r];
         EP.ntiles = bounds[r + 1];
-        const int grid = std::min<int>(e->grid, (int)(bounds[r + 1] - bounds[r]));
+        const int count = (int)(bounds[r + 1] - bounds[r]);
+        const int grid = e->grid == 0 ? count : std::min<int>(e->grid, count);
         ECHK(e, launch_ensemble(e->dtype, e->cpt, kinds[r], pattern, EP, grid, e->block, e->stream));
     }
     return IMEXLB_OK;
@@ -602,7 +608,7 @@ int imexlb_ensemble_info(imexlb_ensemble *e, int64_t *out)
     if (!e || !out) return IMEXLB_EINVAL;
     out[0] = e->cells;
     out[1] = (int64_t)e->tiles.size();
-    out[2] = std::min<int64_t>(e->grid, (int64_t)e->tiles.size());
+    out[2] = e->grid == 0 ? (int64_t)e->tiles.size() : std::min<int64_t>(e->grid, (int64_t)e->tiles.size());
     out[3] = e->block;
     out[4] = e->cpt;
     out[5] = e->device_bytes;
@@ -612,7 +618,7 @@ int imexlb_ensemble_info(imexlb_ensemble *e, int64_t *out)
 
 int imexlb_ensemble_set_launch(imexlb_ensemble *e, int block, int cpt, int grid)
 {
-    if (!e || block < 64 || block > 512 || block % 64 || (cpt != 1 && cpt != 2) || grid < 1) return IMEXLB_EINVAL;
+    if (!e || block < 64 || block > 512 || block % 64 || (cpt != 1 && cpt != 2) || grid < 0) return IMEXLB_EINVAL;
     e->block = block;
     e->cpt = cpt;
     e->grid = grid;
@@ -804,7 +810,9 @@ int imexlb_ensemble_autotune(imexlb_ensemble *e, const char *cache_path, int ste
         steps += steps & 1;
         best.ms = 1e30f;
         const int blocks[3] = {128, 256, 512};
-        const int grids[3] = {e->cus * 4, e->cus * 8, e->cus * 16};
+        // persistent grids of 4 to 64 workgroups per CU, and 0: one
+        // workgroup per tile
+        const int grids[6] = {e->cus * 4, e->cus * 8, e->cus * 16, e->cus * 32, e->cus * 64, 0};
         for (int b : blocks)
             for (int c = 1; c <= 2; ++c)
                 for (int g : grids) {

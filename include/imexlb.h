@@ -299,9 +299,12 @@ int imexlb_ensemble_get_moments(imexlb_ensemble *e, int domain, double *p, doubl
 int imexlb_ensemble_status(imexlb_ensemble *e, int32_t *finite, int32_t *unstable);
 /* out[7]: cells, tiles, grid, block, cells per thread, device bytes, steps */
 int imexlb_ensemble_info(imexlb_ensemble *e, int64_t *out);
+/* grid: workgroups of each persistent launch (each walks every grid-th tile
+ * of its run); 0 = one workgroup per tile. */
 int imexlb_ensemble_set_launch(imexlb_ensemble *e, int block, int cells_per_thread, int grid);
-/* As imexlb_autotune, over block x cells per thread x persistent grid size,
- * timed with graph replay; chosen[3] = block, cells per thread, grid. */
+/* As imexlb_autotune, over block x cells per thread x grid (4 to 64
+ * workgroups per CU, or one per tile), timed with graph replay; chosen[3] =
+ * block, cells per thread, grid. */
 int imexlb_ensemble_autotune(imexlb_ensemble *e, const char *cache_path, int steps, int32_t *chosen);
 /* As imexlb_tune_placement, for the ensemble's lattices, which share one
  * allocation: up to trials - 1 fresh allocations of the whole set are timed

@@ -57,7 +57,8 @@ def test_ensemble_launch_shapes_agree():
     from imexlb_amd import Ensemble
     doms = small_domains()
     ref = None
-    for block, cpt, grid in [(256, 1, 2048), (128, 2, 64), (512, 1, 7), (64, 2, 1)]:
+    # grid 0: one workgroup per tile
+    for block, cpt, grid in [(256, 1, 2048), (128, 2, 64), (512, 1, 7), (64, 2, 1), (256, 2, 0)]:
         e = Ensemble(doms, dtype="f64", storage="aa")
         e.set_launch(block, cpt, grid)
         e.Initialize()
