@@ -60,8 +60,9 @@ def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None, pl
         log(f"[bench] {dtype}: placement GLUPS {rates}, kept {kept}")
     elif placement and placement[0] == "window":
         # the rate follows the lattice's base address smoothly: slide it
-        # through one allocation of lattice + slack and keep the fastest
-        # offset (the handle holds the window)
+        # through one address range of lattice + slack and keep the fastest
+        # offset (the library then gives back the chunks the lattice does not
+        # touch)
         _, slack, step = placement
         kept, ms_each = g.tune_placement_window(slack_gib=slack, step_gib=step, steps=10)
         rates = [round(n ** 3 / (m * 1e-3) / 1e9, 3) if m > 0 else None for m in ms_each]
@@ -254,7 +255,7 @@ def main():
                          "separate allocations, or the first allocation")
     ap.add_argument("--window-slack-gib", type=float, default=96.0,
                     help="window beyond the lattice for --placement window; the fast offsets recur irregularly, "
-                         "about 30-60 GiB apart (the handle keeps the window)")
+                         "about 30-60 GiB apart (held during the search only)")
     ap.add_argument("--placements", type=int, default=None,
                     help="time this many separate allocations instead (1: the first allocation only)")
     ap.add_argument("--placement-budget-gb", type=float, default=128.0,

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/imexlb.h"
+#include "window.h"
 #include "kernels.h"
 #include "phased.h"
 
@@ -25,10 +26,9 @@ struct imexlb_handle {
     bool own_stream = false;
     // fused engine
     void *lat[2] = {nullptr, nullptr};
-    // set by imexlb_tune_placement_window: lat[] lie in this one allocation,
-    // of window_bytes bytes (lattice(s) plus the search's slack)
-    void *window = nullptr;
-    size_t window_bytes = 0;
+    // set by imexlb_tune_placement_window: lat[] lie in this address range,
+    // of which only the chunks under the lattice(s) are still held
+    imx::ChunkedWindow win;
     int cur = 0;             // pull: lattice holding the current state
     double *mom[4] = {nullptr, nullptr, nullptr, nullptr};
     uint8_t *mask = nullptr;
@@ -500,7 +500,7 @@ int imexlb_destroy(imexlb_handle *h)
     if (!h) return IMEXLB_OK;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    if (h->window) hipFree(h->window);
+    if (!h->win.empty()) imx::window_destroy(h->win);
     else
         for (auto &p : h->lat)
             if (p) hipFree(p);
@@ -1009,10 +1009,8 @@ int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per
     const size_t lat_bytes = (size_t)Q * h->pstride * h->elem;
     const int nlat = h->cfg.storage == IMEXLB_PULL ? 2 : 1;
     // candidate 0 is the current lattice, which may lie in a window
-    void *const win0 = h->window;
-    const size_t win0_bytes = h->window_bytes;
-    h->window = nullptr;
-    h->window_bytes = 0;
+    imx::ChunkedWindow win0 = std::move(h->win);
+    h->win = imx::ChunkedWindow{};
     std::vector<std::array<void *, 2>> cand{{h->lat[0], h->lat[1]}};
     std::vector<float> ms(trials, 0.0f);
     auto time_current = [&](float &out) -> int {
@@ -1047,9 +1045,9 @@ int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per
     (void)hipStreamSynchronize(h->stream);
     for (int t = 0; t < (int)cand.size(); ++t) {
         if (t == best) continue;
-        if (t == 0 && win0) {
-            h->device_bytes -= (int64_t)win0_bytes - (int64_t)(nlat * lat_bytes);
-            hipFree(win0);
+        if (t == 0 && !win0.empty()) {
+            h->device_bytes -= (int64_t)win0.held() - (int64_t)(nlat * lat_bytes);
+            imx::window_destroy(win0);
             continue;
         }
         for (void *q : cand[t])
@@ -1057,10 +1055,7 @@ int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per
     }
     h->lat[0] = cand[best][0];
     h->lat[1] = cand[best][1];
-    if (best == 0) {
-        h->window = win0;
-        h->window_bytes = win0_bytes;
-    }
+    if (best == 0) h->win = std::move(win0);
     const int rc2 = imexlb_initialize(h);
     if (rc) return rc;
     if (ms_per_step)
@@ -1092,13 +1087,14 @@ int imexlb_tune_placement_window(imexlb_handle *h, int64_t slack_bytes, int64_t 
     HIPCHK(h, hipMemGetInfo(&free_b, &total_b));
     if (free_b < total + (2ull << 30))
         return fail(h, IMEXLB_ENOMEM, "window of %zu bytes does not fit (%zu free)", total, free_b);
-    // one ordinary allocation, kept whole (giving the slack back would need
-    // memory mapped in chunks and released after kernels used it)
-    void *big = nullptr;
-    if (hipMalloc(&big, total) != hipSuccess) {
+    // one address range in chunks of physical memory: the chunks the kept
+    // offset does not use are given back after the search
+    imx::ChunkedWindow w;
+    if (imx::window_create(w, h->device, total, (size_t)step_bytes) != hipSuccess) {
         (void)hipGetLastError();
         return fail(h, IMEXLB_ENOMEM, "window of %zu bytes", total);
     }
+    char *const big = w.base;
     void *old_lat[2] = {h->lat[0], h->lat[1]};
     std::vector<float> ms((size_t)npos, 0.0f);
     for (int64_t k = 0; k < npos && !rc; ++k) {
@@ -1113,7 +1109,7 @@ int imexlb_tune_placement_window(imexlb_handle *h, int64_t slack_bytes, int64_t 
     if (rc) {  // keep the lattice the handle had
         h->lat[0] = old_lat[0];
         h->lat[1] = old_lat[1];
-        hipFree(big);
+        imx::window_destroy(w);
         const int rc2 = imexlb_initialize(h);
         return rc ? rc : rc2;
     }
@@ -1122,17 +1118,18 @@ int imexlb_tune_placement_window(imexlb_handle *h, int64_t slack_bytes, int64_t 
         if (ms[k] < ms[best]) best = k;
     h->lat[0] = (char *)big + best * step_bytes;
     h->lat[1] = nlat == 2 ? (char *)h->lat[0] + lat_bytes : nullptr;
-    if (h->window) {
-        h->device_bytes -= (int64_t)h->window_bytes;
-        hipFree(h->window);
+    if (!h->win.empty()) {
+        h->device_bytes -= (int64_t)h->win.held();
+        imx::window_destroy(h->win);
     } else {
         h->device_bytes -= (int64_t)(nlat * lat_bytes);
         for (void *q : old_lat)
             if (q) hipFree(q);
     }
-    h->window = big;
-    h->window_bytes = total;
-    h->device_bytes += (int64_t)total;
+    // the lattice stays on the pages it was timed on; the rest goes back
+    imx::window_keep(w, (size_t)(best * step_bytes), (size_t)nlat * lat_bytes);
+    h->device_bytes += (int64_t)w.held();
+    h->win = std::move(w);
     if (ms_per_pos)
         for (int64_t k = 0; k < npos; ++k) ms_per_pos[k] = ms[k];
     if (chosen) *chosen = (int32_t)best;

@@ -22,6 +22,7 @@
 
 #include "../../include/imexlb.h"
 #include "fused.h"
+#include "window.h"
 #include "kernels.h"
 
 namespace imx {
@@ -137,8 +138,7 @@ struct imexlb_ensemble {
     uint32_t ntiles_periodic = 0;
     uint2 *dtiles = nullptr;
     void *pool = nullptr;                // every lattice, in one allocation
-    void *pool_alloc = nullptr;          // the allocation pool lies in when a window search placed it
-    size_t window_bytes = 0;             // its size (pool_bytes + slack)
+    ChunkedWindow win;                   // the address range pool lies in when a window search placed it
     size_t pool_bytes = 0;
     std::vector<size_t> lat_off;         // offset of each lattice in the pool
     std::vector<void *> lat;             // 2 per domain (second null for AA)
@@ -347,7 +347,7 @@ int imexlb_ensemble_destroy(imexlb_ensemble *e)
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
-    if (e->pool_alloc) (void)hipFree(e->pool_alloc);
+    if (!e->win.empty()) window_destroy(e->win);
     else if (e->pool) (void)hipFree(e->pool);
     for (double *p : e->mom)
         if (p) (void)hipFree(p);
@@ -639,7 +639,8 @@ int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, fl
     if (steps < 2) steps = 2;
     steps += steps & 1;
     std::vector<void *> cand{e->pool};
-    void *const alloc0 = e->pool_alloc;  // candidate 0 may lie in a window
+    ChunkedWindow win0 = std::move(e->win);  // candidate 0 may lie in a window
+    e->win = ChunkedWindow{};
     std::vector<float> ms(trials, 0.0f);
     auto time_current = [&](float &out) -> int {
         int r = imexlb_ensemble_initialize(e);
@@ -666,12 +667,15 @@ int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, fl
         for (int t = 1; t < (int)cand.size(); ++t)
             if (ms[t] < ms[best]) best = t;
     (void)hipStreamSynchronize(e->stream);
-    for (int t = 0; t < (int)cand.size(); ++t)
-        if (t != best) (void)hipFree(t == 0 && alloc0 ? alloc0 : cand[t]);
-    if (best != 0 && alloc0) {
-        e->device_bytes -= (int64_t)(e->window_bytes - e->pool_bytes);
-        e->pool_alloc = nullptr;
-        e->window_bytes = 0;
+    for (int t = 1; t < (int)cand.size(); ++t)
+        if (t != best) (void)hipFree(cand[t]);
+    if (best == 0) {
+        e->win = std::move(win0);
+    } else if (!win0.empty()) {
+        e->device_bytes -= (int64_t)win0.held() - (int64_t)e->pool_bytes;
+        window_destroy(win0);
+    } else {
+        (void)hipFree(cand[0]);
     }
     int rc2 = use_pool(e, cand[best]);
     if (!rc2) rc2 = imexlb_ensemble_initialize(e);
@@ -703,12 +707,15 @@ int imexlb_ensemble_tune_placement_window(imexlb_ensemble *e, int64_t slack_byte
     ECHK(e, hipMemGetInfo(&free_b, &total_b));
     if (free_b < total + (2ull << 30))
         return efail(e, IMEXLB_ENOMEM, "window of %zu bytes does not fit (%zu free)", total, free_b);
-    void *big = nullptr;
-    if (hipMalloc(&big, total) != hipSuccess) {
+    // one address range in chunks of physical memory: the chunks the kept
+    // offset does not use are given back after the search
+    ChunkedWindow w;
+    if (window_create(w, e->device, total, (size_t)step_bytes) != hipSuccess) {
         (void)hipGetLastError();
         return efail(e, IMEXLB_ENOMEM, "window of %zu bytes", total);
     }
-    void *const old_pool = e->pool, *const old_alloc = e->pool_alloc;
+    char *const big = w.base;
+    void *const old_pool = e->pool;
     std::vector<float> ms((size_t)npos, 0.0f);
     int rc = IMEXLB_OK;
     for (int64_t k = 0; k < npos && !rc; ++k) {
@@ -721,17 +728,24 @@ int imexlb_ensemble_tune_placement_window(imexlb_ensemble *e, int64_t slack_byte
     (void)hipStreamSynchronize(e->stream);
     if (rc) {  // keep the pool the ensemble had
         const int rc1 = use_pool(e, old_pool);
-        (void)hipFree(big);
+        window_destroy(w);
         const int rc2 = rc1 ? rc1 : imexlb_ensemble_initialize(e);
         return rc ? rc : rc2;
     }
     int64_t best = 0;
     for (int64_t k = 1; k < npos; ++k)
         if (ms[k] < ms[best]) best = k;
-    (void)hipFree(old_alloc ? old_alloc : old_pool);
-    e->device_bytes += (int64_t)slack_bytes - (int64_t)(old_alloc ? e->window_bytes - e->pool_bytes : 0);
-    e->pool_alloc = big;
-    e->window_bytes = total;
+    if (!e->win.empty()) {
+        e->device_bytes -= (int64_t)e->win.held();
+        window_destroy(e->win);
+    } else {
+        e->device_bytes -= (int64_t)e->pool_bytes;
+        (void)hipFree(old_pool);
+    }
+    // the pool stays on the pages it was timed on; the rest goes back
+    window_keep(w, (size_t)(best * step_bytes), e->pool_bytes);
+    e->device_bytes += (int64_t)w.held();
+    e->win = std::move(w);
     rc = use_pool(e, (char *)big + best * step_bytes);
     if (ms_per_pos)
         for (int64_t k = 0; k < npos; ++k) ms_per_pos[k] = ms[k];

@@ -185,9 +185,9 @@ class LBM:
 
     def tune_placement_window(self, slack_gib=32, step_gib=4, steps=10):
         """Time the step with the lattice at offsets 0, step, 2 step .. slack
-        inside one allocation of the lattice plus `slack_gib` GiB and keep the
-        fastest; the handle then holds the whole window.  Returns (index
-        kept, [ms per step at each offset])."""
+        inside one address range of the lattice plus `slack_gib` GiB and keep
+        the fastest; the chunks of the range the lattice does not touch are
+        given back.  Returns (index kept, [ms per step at each offset])."""
         step = int(step_gib * (1 << 30))
         slack = int(slack_gib * (1 << 30)) // step * step
         n = slack // step + 1
@@ -337,7 +337,7 @@ class Ensemble:
         return ch.value, list(ms)
 
     def tune_placement_window(self, slack_gib=32, step_gib=1, steps=6):
-        """As LBM.tune_placement_window, for the allocation that holds every
+        """As LBM.tune_placement_window, for the pool that holds every
         member's lattices; the ensemble is reset to its initial state."""
         step = int(step_gib * (1 << 30))
         slack = int(slack_gib * (1 << 30)) // step * step

@@ -247,14 +247,15 @@ int imexlb_set_launch(imexlb_handle *h, int block, int cells_per_thread, int xcd
  * The state is re-initialised.  Results do not depend on the placement.
  * Fused engine only; trials in [1, 16]. */
 int imexlb_tune_placement(imexlb_handle *h, int trials, int steps, float *ms_per_step, int32_t *chosen);
-/* Placement search inside one allocation (no counterpart in the reference).
- * The step rate follows the lattice's base address smoothly (rising by up to
- * 20 % over a few GiB), so this allocates the lattice(s) plus slack_bytes
- * once, times `steps` steps with the lattice at each offset k * step_bytes
- * (k = 0 .. slack / step; step a multiple of 2 MiB dividing the slack), keeps
- * the fastest offset and frees the handle's previous lattice.  The handle
- * then holds the whole window -- lattice(s) plus slack -- until it is
- * destroyed or searched again.  ms_per_pos (npos_max entries, may be NULL)
+/* Placement search inside one address range (no counterpart in the
+ * reference).  The step rate follows the lattice's base address smoothly
+ * (rising by up to 20 % over a few GiB), so this maps the lattice(s) plus
+ * slack_bytes once, in chunks of physical memory of about step_bytes, times
+ * `steps` steps with the lattice at each offset k * step_bytes (k = 0 ..
+ * slack / step; step a multiple of 2 MiB dividing the slack), keeps the
+ * fastest offset and frees the handle's previous lattice.  The chunks the
+ * kept lattice does not touch are then given back: the handle holds at most
+ * two chunks beyond its lattice(s).  ms_per_pos (npos_max entries, may be NULL)
  * receives each offset's time per step, *chosen (may be NULL) the index
  * kept.  The state is re-initialised; results do not depend on the offset.
  * Fused engine only; at most 256 offsets. */
@@ -311,10 +312,10 @@ int imexlb_ensemble_autotune(imexlb_ensemble *e, const char *cache_path, int ste
  * with graph-replayed steps, the fastest is kept and the ensemble is reset
  * to its initial state.  trials in [1, 16]. */
 int imexlb_ensemble_tune_placement(imexlb_ensemble *e, int trials, int steps, float *ms_per_step, int32_t *chosen);
-/* As imexlb_tune_placement_window, for the one allocation that holds every
- * member's lattices: the set slides through one allocation of its size plus
- * slack_bytes (graph-replayed steps timed at each offset), the fastest
- * offset is kept and the window is held until destroy or the next search. */
+/* As imexlb_tune_placement_window, for the pool that holds every member's
+ * lattices: the pool slides through one address range of its size plus
+ * slack_bytes (graph-replayed steps timed at each offset), the fastest offset
+ * is kept and the chunks the pool does not touch are given back. */
 int imexlb_ensemble_tune_placement_window(imexlb_ensemble *e, int64_t slack_bytes, int64_t step_bytes, int steps,
                                           float *ms_per_pos, int32_t npos_max, int32_t *chosen);
 const char *imexlb_ensemble_last_error(const imexlb_ensemble *e);

@@ -79,7 +79,8 @@ def test_ensemble_autotune_and_cache(tmp_path):
     e = Ensemble(doms, dtype="f32", storage="pull")
     e.Initialize()
     choice = e.autotune(cache, steps=2)
-    assert choice[0] in (128, 256, 512) and choice[1] in (1, 2) and choice[2] > 0
+    # grid: 4 to 64 persistent workgroups per CU, or 0 (one per tile)
+    assert choice[0] in (128, 256, 512) and choice[1] in (1, 2) and choice[2] >= 0
     assert os.path.getsize(cache) > 0
     e2 = Ensemble(doms, dtype="f32", storage="pull")
     e2.Initialize()
@@ -129,17 +130,20 @@ def test_ensemble_placement_keeps_results(storage):
                 for a, b in zip(e.moments(k), ref[k]):
                     assert np.array_equal(a, b), (storage, mode, k)
     check()
-    # the window search: the set slides through one allocation of its size
-    # plus 6 MiB in 2 MiB steps, twice, then a separate-allocation search
-    # starts from the window
+    # the window search: the set slides through one range of its size plus
+    # 6 MiB in 2 MiB steps, twice, then a separate-allocation search starts
+    # from the window.  After a window search the ensemble holds only the
+    # 2 MiB chunks under its pool: less than two chunks beyond the pool
     for rep in range(2):
         kept, ms = e.tune_placement_window(slack_gib=6 / 1024, step_gib=2 / 1024, steps=4)
         print(f"[ensemble window] {storage} {rep}: kept {kept} of ms/step {ms}", flush=True)
         assert len(ms) == 4 and 0 <= kept < 4 and ms[kept] == min(ms)
-        assert e.info()["device_bytes"] == bytes0 + (6 << 20)
+        extra = e.info()["device_bytes"] - bytes0
+        assert 0 <= extra < 2 * (2 << 20), extra
         check()
+    held = e.info()["device_bytes"]
     kept, ms = e.tune_placement(trials=2, steps=4)
-    assert e.info()["device_bytes"] == (bytes0 + (6 << 20) if kept == 0 else bytes0)
+    assert e.info()["device_bytes"] == (held if kept == 0 else bytes0)
     check()
 
 
