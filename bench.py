@@ -63,14 +63,21 @@ def run_gpu(n, steps, warmup, storage, dtype, u0, re, tune=True, launch=None, pl
         # through one address range of lattice + slack and keep the fastest
         # offset (the library then gives back the chunks the lattice does not
         # touch)
+        from imexlb_amd import ImexlbError
         _, slack, step = placement
-        kept, ms_each = g.tune_placement_window(slack_gib=slack, step_gib=step, steps=10)
-        rates = [round(n ** 3 / (m * 1e-3) / 1e9, 3) if m > 0 else None for m in ms_each]
-        held = g.info()["device_bytes"] / 1e9
-        info_p = {"method": "window", "slack_gib": slack, "step_gib": step, "glups_each": rates, "kept": kept,
-                  "held_gb_during_search": round(lat_gb + slack * 2 ** 30 / 1e9, 1),
-                  "held_gb_after": round(held, 1)}
-        log(f"[bench] {dtype}: window GLUPS {rates}, kept offset {kept} ({rates[kept]})")
+        try:
+            kept, ms_each = g.tune_placement_window(slack_gib=slack, step_gib=step, steps=10)
+        except ImexlbError as err:
+            # the library keeps the lattice it had: time the first allocation
+            log(f"[bench] {dtype}: window search failed ({err}); timing the first allocation")
+            kept = None
+        if kept is not None:
+            rates = [round(n ** 3 / (m * 1e-3) / 1e9, 3) if m > 0 else None for m in ms_each]
+            held = g.info()["device_bytes"] / 1e9
+            info_p = {"method": "window", "slack_gib": slack, "step_gib": step, "glups_each": rates, "kept": kept,
+                      "held_gb_during_search": round(lat_gb + slack * 2 ** 30 / 1e9, 1),
+                      "held_gb_after": round(held, 1)}
+            log(f"[bench] {dtype}: window GLUPS {rates}, kept offset {kept} ({rates[kept]})")
     if tune:
         # launch shape from the in-project autotuner, on the placement kept
         # above (re-initialises the state)
