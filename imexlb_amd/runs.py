@@ -25,7 +25,7 @@ ensemble: 256 domains from seed 20261202 (TGV / cavity, edges 32..96,
     replay vs persistent launches vs one launch per domain; per-domain
     velocity error against the same ensemble in FP64.
 Every run but timing places its lattices with the window search (the
-fastest offset of the lattice inside one allocation of lattice + 32-64 GiB);
+fastest offset of the lattice inside one address range of lattice + 96 GiB);
 --placements N times N separate allocations instead, --placements 1 keeps the
 first.
 timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, a fixed launch
@@ -68,8 +68,8 @@ def l2(a, b):
 def place(g, a, cells, label):
     """Lattice placement search before a timed run: with --placements N > 1,
     the fastest of N separate allocations; by default the fastest offset of
-    a window search (the lattice slid through one allocation of lattice + 64
-    GiB, 32 GiB for lattices under 16 GiB, which the handle keeps).
+    a window search (the lattice slid through one address range of lattice
+    + 96 GiB, whose unused chunks are given back afterwards).
     --placements 1: the first allocation.  Returns the record, None if no
     search ran."""
     def rate(m):
@@ -82,7 +82,8 @@ def place(g, a, cells, label):
     else:
         lat_gib = g.info()["device_bytes"] / 2 ** 30
         step = 2 if lat_gib >= 16 else (1 if lat_gib >= 4 else 0.5)
-        slack = 64 if lat_gib >= 16 else 32
+        # the fast offsets recur irregularly, 30-60 GiB apart
+        slack = 96
         kept, ms = g.tune_placement_window(slack_gib=slack, step_gib=step, steps=10)
         rec = {"method": "window", "slack_gib": slack, "step_gib": step, "kept": kept,
                "glups_each": [rate(m) for m in ms]}
@@ -223,8 +224,8 @@ def ensemble(a):
             out["placement"] = {"method": "separate allocations", "kept": kept,
                                 "glups_each": [rate(m) for m in ms_each]}
     else:
-        kept, ms_each = e.tune_placement_window(slack_gib=64, step_gib=2, steps=6)
-        out["placement"] = {"method": "window", "slack_gib": 64, "step_gib": 2, "kept": kept,
+        kept, ms_each = e.tune_placement_window(slack_gib=96, step_gib=2, steps=6)
+        out["placement"] = {"method": "window", "slack_gib": 96, "step_gib": 2, "kept": kept,
                             "glups_each": [rate(m) for m in ms_each]}
     if out.get("placement"):
         log(f"[ensemble] placement: {out['placement']}")
