@@ -58,8 +58,9 @@ __device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t
     return ld_ptr<S, NT>(lat + at, cls);
 }
 
-// How the kernels address a population (the launch picks the mode from the
-// population stride; every mode gives the same results):
+// How the kernels address a population (the launch picks the most compact
+// mode the population stride allows, capped by the handle's configuration;
+// every mode gives the same results):
 // - AM_WIDE: 64-bit element offsets.
 // - AM_W32: population i's array as a uniform base plus the cell's byte
 //   offset as a 32-bit value.  base + zext(offset) is the global saddr form,
@@ -68,12 +69,12 @@ __device__ __forceinline__ typename RealOf<S>::type ld_at(const S *lat, uint64_t
 //   other steps.  Needs one population within 4 GiB.
 // - AM_GROUP: as AM_W32, and where every population is addressed at the same
 //   cell (the AA even step, the pull step's stores) the populations are taken
-//   in groups of POP_GROUP (16 bytes of population stride per cell): one
+//   in groups of PopGroup<S>::n (16 bytes of population stride per cell): one
 //   uniform base per group, and the population's place in the group added to
 //   the 32-bit offset.  With a base per population there the compiler forms
 //   lattice + offset once in VGPRs and adds each population's 64-bit offset
 //   in VALU; with the groups the step keeps a few scalar bases (FP16 even
-//   step: 64 instead of 103 VGPRs).  Needs a group within 4 GiB: at most 2^28
+//   step: 72 instead of 103 VGPRs).  Needs a group within 4 GiB: at most 2^28
 //   cells per population.
 enum AddrMode : int { AM_WIDE = 0, AM_W32 = 1, AM_GROUP = 2 };
 
