@@ -5,7 +5,7 @@ line.
     python -m imexlb_amd.runs cavity    [--n 128] [--steps 20000] [--placements N]
     python -m imexlb_amd.runs sphere    [--steps 50000] [--placements N]
     python -m imexlb_amd.runs ensemble  [--steps 2000] [--domains 256] [--storage aa|pull] [--placements N]
-    python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16]
+    python -m imexlb_amd.runs timing    [--n 512 | --shape NX,NY,NZ] [--steps 20] [--dtype f16] [--window]
                                         [--storage aa] [--launch B,C,X,N] [--placements 1]
     python -m imexlb_amd.runs phased    [--n 256] [--steps 20]
     python -m imexlb_amd.runs placement [--n 512] [--storage aa] [--dtypes f64,f32,f16] [--windows 2]
@@ -28,9 +28,10 @@ Every run but timing places its lattices with the window search (the
 fastest offset of the lattice inside one address range of lattice + 96 GiB);
 --placements N times N separate allocations instead, --placements 1 keeps the
 first.
-timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, a fixed launch
-    shape (or the autotuner's), `steps` timed steps after 4 warm-up steps;
-    GLUPS and algorithmic GB/s.  The short run to profile one kernel with.
+timing: TGV n^3 or NX x NY x NZ (U0 0.05, Re 1600), one dtype and storage, the first
+    allocation (or --window / --placements N), a fixed launch shape (or the
+    autotuner's, on the kept placement), `steps` timed steps after 4 warm-up
+    steps; GLUPS and algorithmic GB/s.  The short run to profile one kernel with.
 placement: TGV n^3 in each storage type at a fixed launch shape: the step
     rate with the lattice at every offset of a window search (--windows of
     them, 2 by default, in separate windows: 64 GiB in 2 GiB steps for FP64,
@@ -277,13 +278,13 @@ def timing(a):
     g = LBM(*shape, tau, u0=0.05, dtype=a.dtype, storage=a.storage)
     if a.launch:
         g.set_launch(*(int(v) for v in a.launch.split(",")))
-        g.Initialize()
-    else:
-        g.autotune(None, steps=8)
+    g.Initialize()
     out = {"workload": f"TGV {'x'.join(map(str, shape))} Re 1600 {a.dtype} {a.storage}", "placement": None}
-    if a.placements is None:
+    if a.placements is None and not a.window:
         a.placements = 1  # timing keeps the first allocation unless asked
     out["placement"] = place(g, a, shape[0] * shape[1] * shape[2], "timing")
+    if not a.launch:
+        g.autotune(None, steps=8)  # on the kept placement
     g.step(4, moments_every=0)
     ms = g.step_timed(a.steps, moments_every=0)
     info = g.info()
@@ -366,6 +367,8 @@ def main(argv=None):
                     help="separate lattice allocations to try instead of the window search (1: the first "
                          "allocation; timing defaults to 1)")
     ap.add_argument("--shape", default=None, help="nx,ny,nz for timing (default n,n,n)")
+    ap.add_argument("--window", action="store_true",
+                    help="timing: place the lattice with the window search (default: the first allocation)")
     ap.add_argument("--dtypes", default="f64,f32,f16", help="placement: storage types to search")
     ap.add_argument("--windows", type=int, default=2, help="placement: window searches per storage type")
     ap.add_argument("--allocations", type=int, default=4,
