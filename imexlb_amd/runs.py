@@ -310,9 +310,9 @@ def phased(a):
     for name, c in cases.items():
         g = LBM(*c["shape"], c["tau"], u0=c["u0"], case=c["case"], mode="faithful", engine=c["engine"])
         g.Initialize()
-        if c["engine"] == "fused":  # launch shape and placement (the phased engine keeps its own layout)
-            g.autotune(None, steps=8)
+        if c["engine"] == "fused":  # placement, then the launch shape on it (the phased engine keeps its layout)
             out["placement"][name] = place(g, a, c["shape"][0] * c["shape"][1] * c["shape"][2], f"phased {name}")
+            g.autotune(None, steps=8)
         g.Initialize()
         g.step(3)
         ms = g.step_timed(a.steps)
