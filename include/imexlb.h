@@ -226,7 +226,8 @@ int imexlb_bandwidth_probe(int device, int kind, size_t bytes, int reps, double 
 
 /* Launch-shape autotuner (no counterpart in the reference).  Times every
  * workgroup size x cells per thread x XCD tile order x load variant on this
- * handle for `steps` steps each, keeps the fastest and appends it to the
+ * handle for `steps` steps each, times the four fastest again (twice each,
+ * interleaved, 2 x steps steps), keeps the fastest sum and appends it to the
  * JSON-lines cache at cache_path (NULL: no cache; a cache hit skips the
  * timing).  All shapes give bit-identical results.  The handle is
  * re-initialised after tuning.  chosen[4] (may be NULL) receives block, cells
