@@ -53,6 +53,28 @@ def test_ensemble_members_step_as_alone(dtype, storage, mode):
                 assert np.array_equal(a, b), (k, nsteps)
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_ensemble_of_one_boundary_kind(kind, storage):
+    """An ensemble of TGV boxes only, or of cavities only: each step runs one
+    launch (the other kind's run of the tile table is empty) and the members
+    still step as alone, in graph replay and with one workgroup per tile."""
+    from imexlb_amd import Ensemble
+    doms = [d for d in small_domains() + small_domains()[::-1] if d["case_id"] == kind]
+    assert len(doms) >= 4
+    for grid in (2048, 0):
+        e = Ensemble(doms, dtype="f64", storage=storage)
+        e.set_launch(256, 1, grid)
+        e.Initialize()
+        singles = [single(d, "f64", storage) for d in doms]
+        e.step(9, mode="graph")
+        for k, g in enumerate(singles):
+            g.step(9)
+            for a, b in zip(e.moments(k), g.moments()):
+                assert np.array_equal(a, b), (kind, storage, grid, k)
+            g.close()
+
+
 def test_ensemble_launch_shapes_agree():
     from imexlb_amd import Ensemble
     doms = small_domains()
