@@ -133,6 +133,24 @@ int LBM::tune_placement(int trials)
     check(imexlb_tune_placement(h_, trials, 10, nullptr, &kept));
     return kept;
 }
+int LBM::tune_placement_window(double slack_gib, int *npos)
+{
+    // steps of 2 GiB for lattices of 16 GiB and more, 1 GiB from 4 GiB, 2 MiB
+    // multiples down to 64 MiB below; at most 256 offsets
+    const imexlb_info i = info();
+    const double lat = (double)i.device_bytes;
+    const int64_t mib2 = 2ll << 20;
+    int64_t step = lat >= 16.0 * (1 << 30) ? 2ll << 30 : lat >= 4.0 * (1 << 30) ? 1ll << 30 : 64ll << 20;
+    int64_t slack = (int64_t)(slack_gib * (double)(1ll << 30)) / mib2 * mib2;
+    while (slack / step > 255) step *= 2;
+    slack = slack / step * step;
+    const int n = (int)(slack / step) + 1;
+    std::vector<float> ms((size_t)n, 0.0f);
+    int32_t kept = 0;
+    check(imexlb_tune_placement_window(h_, slack, step, 10, ms.data(), n, &kept));
+    if (npos) *npos = n;
+    return kept;
+}
 void LBM::checkpoint(const std::string &path) { check(imexlb_checkpoint_write(h_, path.c_str())); }
 void LBM::restart(const std::string &path) { check(imexlb_checkpoint_read(h_, path.c_str())); }
 imexlb_info LBM::info()

@@ -75,6 +75,9 @@ struct LBM {
     void diagnostics(double &ke, double &mass);
     void autotune(const char *cache);
     int tune_placement(int trials);  // index of the lattice kept
+    // window placement search over lattice + slack_gib GiB; *npos receives
+    // the number of offsets timed; returns the index kept
+    int tune_placement_window(double slack_gib, int *npos);
     void checkpoint(const std::string &path);
     void restart(const std::string &path);
     imexlb_info info();

@@ -14,6 +14,9 @@
 //          [--restart FILE] (resume from a restart file; no warm-up)
 //          [--checkpoint FILE] (write a restart file after the last step)
 //          [--place N] (time N lattice allocations, keep the fastest)
+//          [--window GIB] (slide the lattice through an address range of
+//                          lattice + GIB and keep the fastest offset; the
+//                          rest of the range is given back)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -46,6 +49,7 @@ int main(int argc, char **argv)
     bool honour_input = false, fast = false, have_sphere = false, output = false;
     std::string tune, restart, checkpoint;
     int place = 1;
+    double window_gib = 0.0;
     for (int a = 1; a < argc; ++a) {
         std::string k = argv[a];
         auto val = [&]() -> std::string {
@@ -71,6 +75,7 @@ int main(int argc, char **argv)
         else if (k == "--restart") restart = val();
         else if (k == "--checkpoint") checkpoint = val();
         else if (k == "--place") place = std::atoi(val().c_str());
+        else if (k == "--window") window_gib = std::atof(val().c_str());
         else if (k == "--sphere") {
             std::string s = val();
             if (std::sscanf(s.c_str(), "%lf,%lf,%lf,%lf", &opt.sphere[0], &opt.sphere[1], &opt.sphere[2],
@@ -105,13 +110,19 @@ int main(int argc, char **argv)
         }
         CommHelper comm(1, 0, opt.case_id == IMEXLB_CASE_TGV, s1.sx, s1.sy, s1.sz);
         LBM l1(comm, s1.sx, s1.sy, s1.sz, s1.tau, s1.rho0, s1.u0, opt);
-        if (opt.engine == IMEXLB_ENGINE_PHASED && (!tune.empty() || place > 1)) {
-            std::cerr << "note: --tune and --place apply to the fused engine; the phased engine keeps the "
-                         "reference's layout" << std::endl;
+        if (opt.engine == IMEXLB_ENGINE_PHASED && (!tune.empty() || place > 1 || window_gib > 0)) {
+            std::cerr << "note: --tune, --place and --window apply to the fused engine; the phased engine keeps "
+                         "the reference's layout" << std::endl;
         } else {
-            if (!tune.empty()) l1.autotune(tune.c_str());
-            if (place > 1)
+            // placement first, then the launch shape on the kept lattice
+            if (window_gib > 0) {
+                int npos = 0;
+                const int kept = l1.tune_placement_window(window_gib, &npos);
+                std::cout << "Lattice placement window: kept offset " << kept << " of " << npos << std::endl;
+            } else if (place > 1) {
                 std::cout << "Lattice placement kept: " << l1.tune_placement(place) << " of " << place << std::endl;
+            }
+            if (!tune.empty()) l1.autotune(tune.c_str());
         }
         l1.Initialize();
         l1.setup_subdomain();

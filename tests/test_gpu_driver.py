@@ -95,6 +95,21 @@ def test_driver_fast_path(tmp_path):
     assert_diag((ke4, m4), oracle_diagnostics(tmp_path, "cavity", "faithful", 13))
 
 
+def test_driver_window_placement(tmp_path):
+    """--window places the lattice with the window search (then --tune picks
+    the launch shape on it): the run ends where the run without it ends."""
+    text = "1.0  0.1  100\n0.05  20  10\n32 32 32\n"
+    ke1, m1, _ = run(tmp_path, text, ["--case", "tgv", "--honour-input", "--storage", "aa"])
+    ke2, m2, out = run(tmp_path, text, ["--case", "tgv", "--honour-input", "--storage", "aa", "--window", "0.25",
+                                        "--tune", str(tmp_path / "tune.jsonl")])
+    m = re.search(r"Lattice placement window: kept offset (\d+) of (\d+)", out)
+    assert m and int(m.group(2)) == 5 and 0 <= int(m.group(1)) < 5, out
+    assert (ke1, m1) == (ke2, m2)
+    text = "1.0  0.1  10\n0.1  13  10\n20 16 12\n"
+    ke3, m3, _ = run(tmp_path, text, ["--case", "cavity", "--honour-input", "--window", "0.25"])
+    assert_diag((ke3, m3), oracle_diagnostics(tmp_path, "cavity", "faithful", 13))
+
+
 @pytest.mark.parametrize("storage", ["pull", "aa"])
 def test_driver_restart_continues_the_run(tmp_path, storage):
     """30 steps, a restart file, then 11 more steps from it, end where one run
