@@ -159,7 +159,7 @@ static int validate(const imexlb_config *c)
         return fail(nullptr, IMEXLB_EINVAL, "block %d must be a multiple of 64 in [64, 512]", c->block);
     if (c->stride_pad < -1 || c->stride_pad > (1 << 24) || (c->stride_pad > 0 && c->stride_pad % 256))
         return fail(nullptr, IMEXLB_EINVAL, "stride_pad %d must be -1 or a multiple of 256 below 16 MiB", c->stride_pad);
-    if (c->cells_per_thread < 0 || c->cells_per_thread > 2)
+    if (c->cells_per_thread < 0 || c->cells_per_thread > 3)
         return fail(nullptr, IMEXLB_EINVAL, "cells_per_thread %d", c->cells_per_thread);
     if (c->addr_mode < -1 || c->addr_mode > 2) return fail(nullptr, IMEXLB_EINVAL, "addr_mode %d", c->addr_mode);
     if (c->engine == IMEXLB_ENGINE_FUSED) {
@@ -964,8 +964,13 @@ int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t
         };
         std::vector<TuneChoice> cand;
         const int blocks[4] = {64, 128, 256, 512};
+        // cell pairs (3) only where their kernel runs (periodic, nx even,
+        // grouped bases: address mode 2); elsewhere they would time the
+        // one-cell kernel again
+        const int am = fused_addr_mode(h->elem, h->pstride, c.addr_mode < 0 ? 2 : c.addr_mode);
+        const int max_cpt = bc_of(c) == BC_PERIODIC && c.nx % 2 == 0 && am == 2 ? 3 : 2;
         for (int b : blocks)
-            for (int cpt = 1; cpt <= 2; ++cpt)
+            for (int cpt = 1; cpt <= max_cpt; ++cpt)
                 for (int xcd = 0; xcd <= 1; ++xcd)
                     for (int ntl = 0; ntl <= 1; ++ntl) {
                         TuneChoice t{b, cpt, xcd, ntl, 0.0f};
@@ -1007,7 +1012,7 @@ int imexlb_set_launch(imexlb_handle *h, int block, int cpt, int xcd, int ntl)
 {
     if (!h) return IMEXLB_EINVAL;
     if (h->ph) return fail(h, IMEXLB_EUNSUPPORTED, "the phased engine has no launch shapes");
-    if (block < 64 || block > 512 || block % 64 || (cpt != 1 && cpt != 2) || (xcd != 0 && xcd != 1) ||
+    if (block < 64 || block > 512 || block % 64 || (cpt < 1 || cpt > 3) || (xcd != 0 && xcd != 1) ||
         (ntl != 0 && ntl != 1))
         return fail(h, IMEXLB_EINVAL, "launch shape %d/%d/%d/%d", block, cpt, xcd, ntl);
     h->block = block;

@@ -633,6 +633,176 @@ __global__ void __launch_bounds__(512) k_fused(const FusedParams P)
     }
 }
 
+// Two adjacent cells per thread: cells c and c + 1 with c = 2 t, x even, on
+// a periodic lattice with nx even (the pair never leaves its row).  Every
+// population of the pair moves as one access of 2 sizeof(S) bytes: a
+// wavefront covers 128 cells of a row, so a shifted row segment splits fewer
+// cache lines at its ends than two segments of 64 cells.  Only the pair at a
+// row's start or end, whose shifted neighbours wrap to (nx - 1, 0), needs the
+// element x = 0 separately: it is loaded in a block that only wavefronts
+// holding such a pair enter, and stored by those lanes alone.
+template <typename S>
+struct PairOf {
+    // two elements at the element's own alignment (shifted pairs start at
+    // odd x)
+    typedef S type __attribute__((ext_vector_type(2), aligned(sizeof(S))));
+};
+
+template <typename S, bool NT>
+__device__ __forceinline__ void ld_pair(const S *ptr, int cls, typename RealOf<S>::type &a,
+                                        typename RealOf<S>::type &b)
+{
+    using V = typename PairOf<S>::type;
+    const auto *p = (const __attribute__((address_space(1))) V *)ptr;
+    V v;
+    if constexpr (NT) v = __builtin_nontemporal_load(p);
+    else v = *p;
+    if constexpr (std::is_same<S, _Float16>::value) {
+        a = (float)v.x * H_INV_SCALE[cls];
+        b = (float)v.y * H_INV_SCALE[cls];
+    } else {
+        a = v.x;
+        b = v.y;
+    }
+}
+
+template <typename S, bool NT>
+__device__ __forceinline__ void st_pair(S *ptr, int cls, typename RealOf<S>::type a, typename RealOf<S>::type b)
+{
+    using V = typename PairOf<S>::type;
+    auto *p = (__attribute__((address_space(1))) V *)ptr;
+    V v;
+    if constexpr (std::is_same<S, _Float16>::value) {
+        v.x = (_Float16)(a * H_SCALE[cls]);
+        v.y = (_Float16)(b * H_SCALE[cls]);
+    } else {
+        v.x = a;
+        v.y = b;
+    }
+    if constexpr (NT && IMX_STORE_POLICY != 1) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+__device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
+
+template <typename S, int PAT, bool NTL>
+__global__ void __launch_bounds__(512) k_fused_pair(const FusedParams P)
+{
+    using R = typename RealOf<S>::type;
+    uint32_t blk = blockIdx.x;
+    if (P.xcd_remap) blk = xcd_tile(blk, P.nblocks);
+    const uint32_t c = 2u * (blk * blockDim.x + threadIdx.x);
+    if (c >= P.ncell) return;  // ncell is even: both cells of a started pair exist
+    const uint32_t nx = P.nx, ny = P.ny, nz = P.nz;
+    const uint32_t t = P.divx.div(c);
+    const uint32_t x = c - t * nx;
+    const uint32_t z = P.divy.div(t);
+    const uint32_t y = t - z * ny;
+    const uint32_t yv[3] = {y == 0 ? ny - 1 : y - 1, y, y == ny - 1 ? 0 : y + 1};
+    const uint32_t zv[3] = {z == 0 ? nz - 1 : z - 1, z, z == nz - 1 ? 0 : z + 1};
+    uint32_t rowb[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) rowb[a][b] = (zv[a] * ny + yv[b]) * nx;
+    // a pair at the row's start: its x - 1 neighbours are (nx - 1, 0); at the
+    // row's end its x + 1 neighbours are (nx - 1, 0)
+    const bool lo = x == 0, hi = x + 2 == nx;
+    const S *src = (const S *)P.src;
+    S *dst = (S *)P.dst;
+    const uint64_t ps = P.pstride;
+
+    R f0[Q], f1[Q];
+    if constexpr (PAT == PAT_AA_EVEN) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) ld_pair<S, NTL>(pop_ptr<const S, AM_GROUP, true>(src, ps, i, c), WCLASS[i], f0[i], f1[i]);
+    } else {
+        // slot s of the pair (x, x + 1) - e_i (pull: s = i; AA odd: s = bb(i)).
+        // The element x = 0 of split pairs is loaded first, raw, in blocks
+        // that only wavefronts holding such a pair enter; the selects and
+        // conversions come after the pairs' loads, so nothing waits inside
+        // the blocks.
+        using V = typename PairOf<S>::type;
+        S fix[Q];
+        if (wave_any(lo)) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                if (EX[i] > 0)
+                    fix[i] = *glb(pop_ptr<const S, AM_GROUP>(src, ps, PAT == PAT_PULL ? i : BB[i],
+                                                             rowb[1 - EZ[i]][1 - EY[i]]));
+        }
+        if (wave_any(hi)) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                if (EX[i] < 0)
+                    fix[i] = *glb(pop_ptr<const S, AM_GROUP>(src, ps, PAT == PAT_PULL ? i : BB[i],
+                                                             rowb[1 - EZ[i]][1 - EY[i]]));
+        }
+        V raw[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int s = PAT == PAT_PULL ? i : BB[i];
+            const uint32_t row = rowb[1 - EZ[i]][1 - EY[i]];
+            const bool split = EX[i] > 0 ? lo : (EX[i] < 0 ? hi : false);
+            // a split pair loads (nx - 2, nx - 1) and keeps the second
+            const uint32_t o = split ? row + nx - 2 : row + x - EX[i];
+            const auto *p = (const __attribute__((address_space(1))) V *)pop_ptr<const S, AM_GROUP>(src, ps, s, o);
+            if (NTL && EX[i] == 0) raw[i] = __builtin_nontemporal_load(p);
+            else raw[i] = *p;
+        }
+        // every load is issued before the first value is used
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            S a = raw[i].x, b = raw[i].y;
+            if (EX[i] != 0) {
+                const bool split = EX[i] > 0 ? lo : hi;
+                a = split ? raw[i].y : a;
+                b = split ? fix[i] : b;
+            }
+            if constexpr (std::is_same<S, _Float16>::value) {
+                f0[i] = (float)a * H_INV_SCALE[WCLASS[i]];
+                f1[i] = (float)b * H_INV_SCALE[WCLASS[i]];
+            } else {
+                f0[i] = a;
+                f1[i] = b;
+            }
+        }
+    }
+
+    CellCtx k;
+    k.c = c;
+    cell_collide<S, BC_PERIODIC>(P, k, f0, P.mom_p != nullptr);
+    k.c = c + 1;
+    cell_collide<S, BC_PERIODIC>(P, k, f1, P.mom_p != nullptr);
+
+    if constexpr (PAT == PAT_PULL) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) st_pair<S, true>(pop_ptr<S, AM_GROUP, true>(dst, ps, i, c), WCLASS[i], f0[i], f1[i]);
+    } else if constexpr (PAT == PAT_AA_EVEN) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) st_pair<S, true>(pop_ptr<S, AM_GROUP, true>(dst, ps, BB[i], c), WCLASS[i], f0[i], f1[i]);
+    } else {
+        // direction i to slot i of the pair (x, x + 1) + e_i
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const uint32_t row = rowb[1 + EZ[i]][1 + EY[i]];
+            if (EX[i] == 0) {
+                st_pair<S, true>(pop_ptr<S, AM_GROUP>(dst, ps, i, row + x), WCLASS[i], f0[i], f1[i]);
+            } else {
+                const bool split = EX[i] > 0 ? hi : lo;
+                if (!split) {
+                    const uint32_t o = EX[i] > 0 ? row + x + 1 : row + x - 1;
+                    st_pair<S, IMX_STORE_POLICY != 2>(pop_ptr<S, AM_GROUP>(dst, ps, i, o), WCLASS[i], f0[i], f1[i]);
+                } else {
+                    st_ptr<S, IMX_STORE_POLICY != 2>(pop_ptr<S, AM_GROUP>(dst, ps, i, row + nx - 1), WCLASS[i], f0[i]);
+                    st_ptr<S, IMX_STORE_POLICY != 2>(pop_ptr<S, AM_GROUP>(dst, ps, i, row), WCLASS[i], f1[i]);
+                }
+            }
+        }
+    }
+}
+
 template <typename S, int CPT, bool NTL, int AM>
 hipError_t launch_fused_cpt(int bc, int pattern, const FusedParams &P, int block, hipStream_t st)
 {
@@ -667,19 +837,35 @@ hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, 
 {
     static_assert(16 / sizeof(S) == (size_t)PopGroup<S>::n, "group of 16 bytes of stride per cell");
     const int am = fused_addr_mode(sizeof(S), P.pstride, P.am_max);
-    if (am == AM_GROUP) {
-        if (P.ntl) {
-            if (P.cpt == 2) return launch_fused_cpt<S, 2, true, AM_GROUP>(bc, pattern, P, block, st);
-            return launch_fused_cpt<S, 1, true, AM_GROUP>(bc, pattern, P, block, st);
-        }
-        if (P.cpt == 2) return launch_fused_cpt<S, 2, false, AM_GROUP>(bc, pattern, P, block, st);
-        return launch_fused_cpt<S, 1, false, AM_GROUP>(bc, pattern, P, block, st);
+    if (am == AM_GROUP && P.cpt == 3 && bc == BC_PERIODIC && P.nx % 2 == 0) {
+        FusedParams P2 = P;
+        P2.nblocks = (P.ncell / 2 + (uint32_t)block - 1) / (uint32_t)block;
+        dim3 grid(P2.nblocks), blk(block);
+#define IMX_L(PT)                                                                 \
+    do {                                                                          \
+        if (P.ntl) hipLaunchKernelGGL((k_fused_pair<S, PT, true>), grid, blk, 0, st, P2); \
+        else hipLaunchKernelGGL((k_fused_pair<S, PT, false>), grid, blk, 0, st, P2);      \
+    } while (0)
+        if (pattern == PAT_PULL) IMX_L(PAT_PULL);
+        else if (pattern == PAT_AA_EVEN) IMX_L(PAT_AA_EVEN);
+        else IMX_L(PAT_AA_ODD);
+#undef IMX_L
+        return hipGetLastError();
     }
-    // larger lattices: one cell per thread (the same results), so the grid
-    // covers block cells per workgroup whatever cells per thread was asked
+    // larger lattices, and pairs where the lattice does not allow them: one
+    // cell per thread (the same results), so the grid covers block cells per
+    // workgroup whatever cells per thread was asked
     FusedParams P1 = P;
     P1.cpt = 1;
     P1.nblocks = (P.ncell + (uint32_t)block - 1) / (uint32_t)block;
+    if (am == AM_GROUP) {
+        if (P.cpt == 2) {
+            if (P.ntl) return launch_fused_cpt<S, 2, true, AM_GROUP>(bc, pattern, P, block, st);
+            return launch_fused_cpt<S, 2, false, AM_GROUP>(bc, pattern, P, block, st);
+        }
+        if (P.ntl) return launch_fused_cpt<S, 1, true, AM_GROUP>(bc, pattern, P1, block, st);
+        return launch_fused_cpt<S, 1, false, AM_GROUP>(bc, pattern, P1, block, st);
+    }
     if (am == AM_W32) {
         if (P.ntl) return launch_fused_cpt<S, 1, true, AM_W32>(bc, pattern, P1, block, st);
         return launch_fused_cpt<S, 1, false, AM_W32>(bc, pattern, P1, block, st);

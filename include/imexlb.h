@@ -77,7 +77,9 @@ typedef struct imexlb_config {
     double sphere_cx, sphere_cy, sphere_cz, sphere_r; /* config-mode sphere */
     uint64_t seed;    /* faithful-mode sphere inlet perturbation */
     int32_t block;    /* threads per workgroup, 0 = default */
-    int32_t cells_per_thread; /* 0 = default */
+    int32_t cells_per_thread; /* 0 = default; 1, 2 (blockDim apart) or 3 (two
+                                 adjacent x cells; periodic lattices with nx
+                                 even, one cell per thread elsewhere) */
     int32_t xcd_remap;        /* -1 = default, 0 off, 1 on */
     int32_t device;           /* HIP device ordinal */
     int32_t stride_pad;       /* bytes added between populations, multiple of
@@ -225,7 +227,8 @@ int imexlb_copy_bandwidth(int device, size_t bytes, int reps, double *gbps, floa
 int imexlb_bandwidth_probe(int device, int kind, size_t bytes, int reps, double *gbps, float *ms_per_launch);
 
 /* Launch-shape autotuner (no counterpart in the reference).  Times every
- * workgroup size x cells per thread x XCD tile order x load variant on this
+ * workgroup size x cells per thread (1, 2, and 3 where the lattice allows
+ * cell pairs) x XCD tile order x load variant on this
  * handle for `steps` steps each, times the four fastest again (twice each,
  * interleaved, 2 x steps steps), keeps the fastest sum and appends it to the
  * JSON-lines cache at cache_path (NULL: no cache; a cache hit skips the
@@ -233,8 +236,9 @@ int imexlb_bandwidth_probe(int device, int kind, size_t bytes, int reps, double 
  * re-initialised after tuning.  chosen[4] (may be NULL) receives block, cells
  * per thread, XCD order, non-temporal loads.  Fused engine only. */
 int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t *chosen);
-/* Set the launch shape by hand (block 64..512 step 64, cells per thread 1|2,
- * XCD order 0|1, non-temporal loads 0|1); results do not depend on it. */
+/* Set the launch shape by hand (block 64..512 step 64, cells per thread 1|2|3
+ * as imexlb_config.cells_per_thread, XCD order 0|1, non-temporal loads 0|1);
+ * results do not depend on it. */
 int imexlb_set_launch(imexlb_handle *h, int block, int cells_per_thread, int xcd_order, int nt_loads);
 /* Lattice placement search (no counterpart in the reference).  The step rate
  * of a lattice depends by several percent on where its pages land in HBM,

@@ -70,6 +70,49 @@ def test_every_launch_shape_gives_identical_bits(storage):
         assert_bitexact(o, g)
 
 
+PAIR_EXTENTS = [(2, 3, 2), (4, 2, 3), (6, 5, 3), (64, 3, 2), (66, 5, 3), (130, 3, 3), (256, 2, 2)]
+PAIR_SHAPES = [(64, 3, 0, 0), (256, 3, 1, 1), (512, 3, 1, 0)]
+
+
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_cell_pairs_bitexact(storage):
+    """Two adjacent cells per thread (cells_per_thread 3) on periodic lattices
+    with nx even, including the pairs at row ends whose x neighbours wrap
+    (nx = 2: every pair): bit for bit the oracle's FP64 run; on odd nx and on
+    the cavity the launch falls back to one cell per thread."""
+    for n in PAIR_EXTENTS + [(67, 3, 2)]:
+        for shape in PAIR_SHAPES:
+            print(f"[case] cell pairs {n} {shape} {storage}", flush=True)
+            o, g = run_pair(TGV, FAITHFUL, n, tgv_tau(n[0]), 0.05, storage, 5, shape=shape)
+            assert_bitexact(o, g)
+    print(f"[case] cell pairs cavity config {storage}", flush=True)
+    o, g = run_pair(LDC, CONFIG, (6, 4, 5), 0.096, 0.0, storage, 4, shape=(64, 3, 1, 1))
+    assert_bitexact(o, g)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["pull", "aa"])
+def test_cell_pairs_reduced_precision_identical(dtype, storage):
+    """FP32/FP16 storage: the pair kernels store the same bits as one cell
+    per thread (the same per-cell arithmetic and conversions)."""
+    from imexlb_amd import LBM
+    for n in PAIR_EXTENTS:
+        print(f"[case] cell pairs {dtype} {n} {storage}", flush=True)
+        tau = tgv_tau(n[0])
+        runs = []
+        for shape in [(256, 1, 1, 0), (128, 3, 1, 1), (64, 3, 0, 0)]:
+            g = LBM(*n, tau, u0=0.05, dtype=dtype, storage=storage)
+            g.set_launch(*shape)
+            g.Initialize()
+            g.step(5)
+            runs.append((g.get_f(), g.moments()))
+            g.close()
+        for f, m in runs[1:]:
+            assert np.array_equal(f, runs[0][0])
+            for a, b in zip(m, runs[0][1]):
+                assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("dtype,tol", [("f32", 2e-5), ("f16", 2e-2)])
 def test_reduced_precision_ragged(dtype, tol):
     """FP32/FP16 storage on a ragged extent stays within the tolerance of the
@@ -95,7 +138,7 @@ def test_live_handle_rejects_bad_calls(tmp_path):
     with pytest.raises(ImexlbError):
         g.step(1)  # before Initialize
     g.Initialize()
-    for bad in [(32, 1, 1, 0), (576, 1, 1, 0), (64, 3, 1, 0), (64, 1, 2, 0), (64, 1, 1, 2)]:
+    for bad in [(32, 1, 1, 0), (576, 1, 1, 0), (64, 4, 1, 0), (64, 0, 1, 0), (64, 1, 2, 0), (64, 1, 1, 2)]:
         with pytest.raises(ImexlbError):
             g.set_launch(*bad)
     with pytest.raises(ValueError):
