@@ -728,15 +728,15 @@ __global__ void __launch_bounds__(512) k_fused_pair(const FusedParams P)
 #pragma unroll
             for (int i = 0; i < Q; ++i)
                 if (EX[i] > 0)
-                    fix[i] = *glb(pop_ptr<const S, AM_GROUP>(src, ps, PAT == PAT_PULL ? i : BB[i],
-                                                             rowb[1 - EZ[i]][1 - EY[i]]));
+                    fix[i] = *glb(pop_ptr<const S, AM_GROUP, true>(src, ps, PAT == PAT_PULL ? i : BB[i],
+                                                                   rowb[1 - EZ[i]][1 - EY[i]]));
         }
         if (wave_any(hi)) {
 #pragma unroll
             for (int i = 0; i < Q; ++i)
                 if (EX[i] < 0)
-                    fix[i] = *glb(pop_ptr<const S, AM_GROUP>(src, ps, PAT == PAT_PULL ? i : BB[i],
-                                                             rowb[1 - EZ[i]][1 - EY[i]]));
+                    fix[i] = *glb(pop_ptr<const S, AM_GROUP, true>(src, ps, PAT == PAT_PULL ? i : BB[i],
+                                                                   rowb[1 - EZ[i]][1 - EY[i]]));
         }
         V raw[Q];
 #pragma unroll
@@ -746,7 +746,7 @@ __global__ void __launch_bounds__(512) k_fused_pair(const FusedParams P)
             const bool split = EX[i] > 0 ? lo : (EX[i] < 0 ? hi : false);
             // a split pair loads (nx - 2, nx - 1) and keeps the second
             const uint32_t o = split ? row + nx - 2 : row + x - EX[i];
-            const auto *p = (const __attribute__((address_space(1))) V *)pop_ptr<const S, AM_GROUP>(src, ps, s, o);
+            const auto *p = (const __attribute__((address_space(1))) V *)pop_ptr<const S, AM_GROUP, true>(src, ps, s, o);
             if (NTL && EX[i] == 0) raw[i] = __builtin_nontemporal_load(p);
             else raw[i] = *p;
         }
@@ -783,21 +783,33 @@ __global__ void __launch_bounds__(512) k_fused_pair(const FusedParams P)
 #pragma unroll
         for (int i = 0; i < Q; ++i) st_pair<S, true>(pop_ptr<S, AM_GROUP, true>(dst, ps, BB[i], c), WCLASS[i], f0[i], f1[i]);
     } else {
-        // direction i to slot i of the pair (x, x + 1) + e_i
+        // direction i to slot i of the pair (x, x + 1) + e_i.  The split
+        // pairs of one x direction are stored in one block per side, so the
+        // lanes' masks change four times per thread, not twice per direction.
+        constexpr bool NTS = IMX_STORE_POLICY != 2;  // shifted stores: see st_ptr
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const uint32_t row = rowb[1 + EZ[i]][1 + EY[i]];
-            if (EX[i] == 0) {
-                st_pair<S, true>(pop_ptr<S, AM_GROUP>(dst, ps, i, row + x), WCLASS[i], f0[i], f1[i]);
+        for (int i = 0; i < Q; ++i)
+            if (EX[i] == 0)
+                st_pair<S, true>(pop_ptr<S, AM_GROUP, true>(dst, ps, i, rowb[1 + EZ[i]][1 + EY[i]] + x), WCLASS[i],
+                                 f0[i], f1[i]);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int ex = side == 0 ? 1 : -1;
+            const bool split = ex > 0 ? hi : lo;
+            if (!split) {
+#pragma unroll
+                for (int i = 0; i < Q; ++i)
+                    if (EX[i] == ex)
+                        st_pair<S, NTS>(pop_ptr<S, AM_GROUP, true>(dst, ps, i, rowb[1 + EZ[i]][1 + EY[i]] + x + ex),
+                                        WCLASS[i], f0[i], f1[i]);
             } else {
-                const bool split = EX[i] > 0 ? hi : lo;
-                if (!split) {
-                    const uint32_t o = EX[i] > 0 ? row + x + 1 : row + x - 1;
-                    st_pair<S, IMX_STORE_POLICY != 2>(pop_ptr<S, AM_GROUP>(dst, ps, i, o), WCLASS[i], f0[i], f1[i]);
-                } else {
-                    st_ptr<S, IMX_STORE_POLICY != 2>(pop_ptr<S, AM_GROUP>(dst, ps, i, row + nx - 1), WCLASS[i], f0[i]);
-                    st_ptr<S, IMX_STORE_POLICY != 2>(pop_ptr<S, AM_GROUP>(dst, ps, i, row), WCLASS[i], f1[i]);
-                }
+#pragma unroll
+                for (int i = 0; i < Q; ++i)
+                    if (EX[i] == ex) {
+                        const uint32_t row = rowb[1 + EZ[i]][1 + EY[i]];
+                        st_ptr<S, NTS>(pop_ptr<S, AM_GROUP, true>(dst, ps, i, row + nx - 1), WCLASS[i], f0[i]);
+                        st_ptr<S, NTS>(pop_ptr<S, AM_GROUP, true>(dst, ps, i, row), WCLASS[i], f1[i]);
+                    }
             }
         }
     }
