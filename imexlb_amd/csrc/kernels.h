@@ -89,7 +89,8 @@ struct FusedParams {
     double feq_in[Q];        // channel: inlet equilibrium, stored convention
     uint32_t nblocks;        // grid size (for the XCD-aware block order)
     int xcd_remap;           // 1: give each XCD one contiguous run of tiles
-    int cpt;                 // cells per thread (1 or 2), blockDim apart
+    int cpt;                 // cells per thread: 1, 2 (blockDim apart) or 3 (two adjacent
+                             // x cells; periodic lattices with nx even, else 1)
     int ntl;                 // 1: non-temporal loads of unshared lines
     int am_max;              // widest-saving address mode allowed (AddrMode in fused.h)
     int bc;                  // boundary kind (read by the ensemble kernel)

@@ -172,6 +172,10 @@ class LBM:
         return tuple(ch)
 
     def set_launch(self, block, cells_per_thread=1, xcd_order=1, nt_loads=0):
+        """Launch shape of the fused kernels (imexlb_set_launch): block 64..512
+        in steps of 64; cells per thread 1, 2 (blockDim apart) or 3 (two
+        adjacent x cells, periodic lattices with nx even; one cell elsewhere);
+        XCD tile order 0/1; non-temporal loads 0/1.  Results do not depend on it."""
         self._chk(lib().imexlb_set_launch(self.h, int(block), int(cells_per_thread), int(xcd_order), int(nt_loads)))
 
     def tune_placement(self, trials=6, steps=10):
