@@ -30,6 +30,28 @@ def test_tgv512_aa_equals_pull_and_conserves_mass():
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("dtype", ["f32", "f16"])
+def test_tgv512_cell_pairs_identical_reduced_precision(dtype):
+    """TGV 512^3, AA, FP32/FP16 storage: the two-adjacent-cell kernel (the
+    bench's FP16 choice) gives the moments of one cell per thread bit for bit
+    after 20 steps, and AA equals pull with pairs too."""
+    from imexlb_amd import LBM
+    n = 512
+    tau = 0.05 * n / 1600.0 / (1 / 3) / 2 / PI_R
+    out = []
+    for storage, shape in [("aa", (256, 1, 1, 1)), ("aa", (256, 3, 0, 1)), ("pull", (128, 3, 1, 1))]:
+        print(f"[tgv512 pairs] {dtype} {storage} {shape}", flush=True)
+        g = LBM(n, n, n, tau, u0=0.05, dtype=dtype, storage=storage)
+        g.set_launch(*shape)
+        g.Initialize()
+        g.step(20)
+        out.append(g.moments())
+        g.close()
+    for m in out[1:]:
+        for name, x, y in zip("puvw", out[0], m):
+            assert np.array_equal(x, y), f"moment {name}"
+
+
 def test_cavity128_matches_oracle_after_1000_steps():
     """The cavity configuration (128^3, Re 400, lid 0.1, halfway bounce-back
     with the lid term) against the oracle, 1000 steps."""
