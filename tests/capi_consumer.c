@@ -30,6 +30,9 @@ int main(int argc, char **argv)
     c.addr_mode = 3; /* no such address mode: refused before any device call */
     CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
     c.addr_mode = -1;
+    c.cells_per_thread = 4; /* 1, 2 or 3 cells per thread (0: default) */
+    CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
+    c.cells_per_thread = 0;
     c.nx = 1; /* extents below 2 are refused before any device call */
     CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
     CHECK(strlen(imexlb_last_error(NULL)) > 0);
