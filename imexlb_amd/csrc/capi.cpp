@@ -931,10 +931,10 @@ int imexlb_get_info(imexlb_handle *h, imexlb_info *info)
     info->bytes_per_lup = 2.0 * Q * (h->ph ? 8.0 : e) + (c.case_id == IMEXLB_CASE_SPHERE && !h->ph ? 1.0 : 0.0);
     info->device_bytes = h->device_bytes;
     info->block = h->block;
-    info->cells_per_thread = h->cpt;
     info->xcd_remap = h->xcd;
     info->nt_loads = h->ntl;
     info->addr_mode = h->ph ? -1 : fused_addr_mode(h->elem, h->pstride, c.addr_mode < 0 ? 2 : c.addr_mode);
+    info->cells_per_thread = h->ph ? h->cpt : fused_cells_per_thread(h->cpt, bc_of(c), (uint32_t)c.nx, info->addr_mode);
     return IMEXLB_OK;
 }
 
@@ -968,7 +968,7 @@ int imexlb_autotune(imexlb_handle *h, const char *cache_path, int steps, int32_t
         // grouped bases: address mode 2); elsewhere they would time the
         // one-cell kernel again
         const int am = fused_addr_mode(h->elem, h->pstride, c.addr_mode < 0 ? 2 : c.addr_mode);
-        const int max_cpt = bc_of(c) == BC_PERIODIC && c.nx % 2 == 0 && am == 2 ? 3 : 2;
+        const int max_cpt = fused_cells_per_thread(3, bc_of(c), (uint32_t)c.nx, am) == 3 ? 3 : 2;
         for (int b : blocks)
             for (int cpt = 1; cpt <= max_cpt; ++cpt)
                 for (int xcd = 0; xcd <= 1; ++xcd)

@@ -849,7 +849,8 @@ hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, 
 {
     static_assert(16 / sizeof(S) == (size_t)PopGroup<S>::n, "group of 16 bytes of stride per cell");
     const int am = fused_addr_mode(sizeof(S), P.pstride, P.am_max);
-    if (am == AM_GROUP && P.cpt == 3 && bc == BC_PERIODIC && P.nx % 2 == 0) {
+    const int cpt = fused_cells_per_thread(P.cpt, bc, P.nx, am);
+    if (cpt == 3) {
         FusedParams P2 = P;
         P2.nblocks = (P.ncell / 2 + (uint32_t)block - 1) / (uint32_t)block;
         dim3 grid(P2.nblocks), blk(block);
@@ -871,7 +872,7 @@ hipError_t launch_fused_t(int bc, int pattern, const FusedParams &P, int block, 
     P1.cpt = 1;
     P1.nblocks = (P.ncell + (uint32_t)block - 1) / (uint32_t)block;
     if (am == AM_GROUP) {
-        if (P.cpt == 2) {
+        if (cpt == 2) {
             if (P.ntl) return launch_fused_cpt<S, 2, true, AM_GROUP>(bc, pattern, P, block, st);
             return launch_fused_cpt<S, 2, false, AM_GROUP>(bc, pattern, P, block, st);
         }

@@ -71,6 +71,17 @@ inline int fused_addr_mode(uint64_t elem, uint64_t pstride, int am_max)
     return 0;
 }
 
+// The cells per thread the fused kernels run with for a requested `cpt`:
+// two adjacent cells (3) only on periodic lattices with nx even and grouped
+// bases (address mode 2), two cells blockDim apart (2) only with grouped
+// bases, else one.
+inline int fused_cells_per_thread(int cpt, int bc, uint32_t nx, int am)
+{
+    if (am != 2) return 1;
+    if (cpt == 3) return bc == BC_PERIODIC && nx % 2 == 0 ? 3 : 1;
+    return cpt == 2 ? 2 : 1;
+}
+
 struct FusedParams {
     int nx, ny, nz;
     uint32_t ncell;          // nx * ny * nz

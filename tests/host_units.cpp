@@ -36,6 +36,20 @@ int main()
     CHECK(fused_addr_mode(8, 1000, -1) == 0);
     std::printf("ok fused_addr_mode\n");
 
+    // the cells per thread a launch runs with: pairs on periodic lattices
+    // with nx even and grouped bases only, two cells with grouped bases only
+    using imx::fused_cells_per_thread;
+    CHECK(fused_cells_per_thread(3, imx::BC_PERIODIC, 512, 2) == 3);
+    CHECK(fused_cells_per_thread(3, imx::BC_PERIODIC, 2, 2) == 3);
+    CHECK(fused_cells_per_thread(3, imx::BC_PERIODIC, 67, 2) == 1);
+    CHECK(fused_cells_per_thread(3, imx::BC_CAVITY, 128, 2) == 1);
+    CHECK(fused_cells_per_thread(3, imx::BC_FAITHFUL, 256, 2) == 1);
+    CHECK(fused_cells_per_thread(3, imx::BC_PERIODIC, 512, 1) == 1);
+    CHECK(fused_cells_per_thread(2, imx::BC_CHANNEL, 512, 2) == 2);
+    CHECK(fused_cells_per_thread(2, imx::BC_PERIODIC, 512, 1) == 1);
+    CHECK(fused_cells_per_thread(1, imx::BC_PERIODIC, 512, 2) == 1);
+    std::printf("ok fused_cells_per_thread\n");
+
     using imx::window_chunk;
     const size_t mib2 = size_t(2) << 20, gib = size_t(1) << 30;
     // the step itself while the window holds at most 512 chunks

@@ -1,5 +1,6 @@
 """Host-side logic of the kernels' launch: which address mode a lattice of a
-given population stride takes, and the chunk size of a placement window.
+given population stride takes, the cells per thread a requested launch
+shape runs with, and the chunk size of a placement window.
 tests/host_units.cpp includes the library's headers and checks the limits
 (2^28 cells per population for the grouped bases, 4 GiB per population for
 per-population bases) at and one past each boundary; it is built with
@@ -23,4 +24,4 @@ def test_address_mode_and_window_chunk_limits(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr + r.stdout
-    assert r.stdout.split() == ["ok", "fused_addr_mode", "ok", "window_chunk"]
+    assert r.stdout.split() == ["ok", "fused_addr_mode", "ok", "fused_cells_per_thread", "ok", "window_chunk"]
