@@ -102,7 +102,9 @@ typedef struct imexlb_info {
     int64_t steps;          /* steps taken since Initialize */
     double bytes_per_lup;   /* algorithmic HBM bytes per cell update */
     int64_t device_bytes;   /* device memory held by the handle */
-    int32_t block, cells_per_thread, xcd_remap, nt_loads;
+    int32_t block, cells_per_thread, xcd_remap, nt_loads; /* launch shape; cells per
+                               thread as the fused kernels run it (1 where the
+                               lattice does not allow the requested 2 or 3) */
     int32_t addr_mode;      /* the fused kernels' address mode; -1 phased engine */
 } imexlb_info;
 
