@@ -616,11 +616,43 @@ __device__ __forceinline__ void faithful_wall(const FusedParams &P, int plane, i
         si = A + sa * e[0]; sj = B + sb * e[1]; sk = src_plane;
         di = A; dj = B; dk = plane;
     }
-    const GhostSrc g = faithful_src(P, bq, si, sj, sk);
-    double v = *g.p;
-    v = g.is_deep ? g.deep : v;
+    double v;
+    if (AXIS == 0 && P.stage != nullptr && sj >= 0 && sj < P.ny && sk >= 0 && sk < P.nz) {
+        // an interior cell of plane 1 or nx - 2: from the copy the last fused
+        // pass wrote, y fastest (one strided cell per row of the lattice
+        // otherwise)
+        const int side = SIGN > 0 ? 0 : 1;
+        v = P.stage[(((uint64_t)side * P.nz + sk) * P.ny + sj) * STAGE_ROW + x_rank(bq)];
+    } else {
+        const GhostSrc g = faithful_src(P, bq, si, sj, sk);
+        v = *g.p;
+        v = g.is_deep ? g.deep : v;
+    }
     const uint64_t r = (uint64_t)ring_index(P, di, dj, dk);
     P.ring[(uint64_t)q * P.ring_n + r] = v;
+}
+
+// FusedParams::stage from the lattice P.src: thread t is entry t of the
+// stage (side, z, y, r).
+__global__ void k_faithful_stage_fill(const FusedParams P)
+{
+    const uint64_t plane = (uint64_t)P.ny * P.nz;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * plane * STAGE_ROW) return;
+    const int r = (int)(t % STAGE_ROW);
+    const uint64_t row = t / STAGE_ROW;  // side * plane + (z ny + y)
+    const int side = row >= plane ? 1 : 0;
+    const uint64_t yz = row - (side ? plane : 0);
+    const int q = side == 0 ? nth_dir<0, -1>(r) : nth_dir<0, 1>(r);
+    const uint32_t x = side == 0 ? 1u : (uint32_t)P.nx - 2u;
+    P.stage[t] = ((const double *)P.src)[(uint64_t)q * P.pstride + yz * P.nx + x];
+}
+
+hipError_t launch_faithful_stage_fill(const FusedParams &P, hipStream_t st)
+{
+    const uint64_t n = 2ull * P.ny * P.nz * STAGE_ROW;
+    hipLaunchKernelGGL(k_faithful_stage_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, P);
+    return hipGetLastError();
 }
 
 // The two walls of one axis in one launch: blockIdx.z = 0 the low wall

@@ -45,6 +45,7 @@ struct imexlb_handle {
     double feq_in[Q] = {};
     double *tgv_tab = nullptr;
     double *ring = nullptr;  // faithful cavity: the first ghost layer (FusedParams::ring)
+    double *stage = nullptr; // faithful cavity: what the x walls read (FusedParams::stage)
     uint32_t ring_a = 0, ring_b = 0, ring_c = 0;
     int block = 256, cpt = 1, xcd = 1, ntl = 0;
     int64_t steps = 0;
@@ -210,6 +211,7 @@ static void fill_fused_params(imexlb_handle *h, FusedParams &P, int src, int dst
     P.divx.init(c.nx);
     P.divy.init(c.ny);
     P.ring = h->ring;
+    P.stage = h->stage;
     if (h->ring) P.ring_rel = ((intptr_t)h->ring - (intptr_t)h->lat[src]) / (intptr_t)sizeof(double);
     P.ring_a = h->ring_a;
     P.ring_b = h->ring_b;
@@ -410,6 +412,11 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
         const size_t rb = (size_t)Q * 2 * (h->ring_a + h->ring_b + h->ring_c) * 8;
         if (hipMalloc(&h->ring, rb) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "ghost ring"));
         h->device_bytes += (int64_t)rb;
+        if (c.case_id == IMEXLB_CASE_CAVITY) {
+            const size_t sb = (size_t)2 * STAGE_ROW * c.ny * c.nz * 8;
+            if (hipMalloc(&h->stage, sb) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "x-wall stage"));
+            h->device_bytes += (int64_t)sb;
+        }
     }
 
     // boundary constants in the oracle's operation order
@@ -511,6 +518,7 @@ int imexlb_destroy(imexlb_handle *h)
     if (h->result) hipFree(h->result);
     if (h->tgv_tab) hipFree(h->tgv_tab);
     if (h->ring) hipFree(h->ring);
+    if (h->stage) hipFree(h->stage);
     if (h->ph) phased_destroy(h->ph);
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
@@ -558,6 +566,7 @@ int imexlb_initialize(imexlb_handle *h)
         FusedParams F;
         fill_fused_params(h, F, 0, 0);
         HIPCHK(h, launch_faithful_ring_init(F, h->stream));
+        if (h->stage) HIPCHK(h, launch_faithful_stage_fill(F, h->stream));
     }
     h->cur = 0;
     if (c.storage == IMEXLB_PULL && c.case_id == IMEXLB_CASE_SPHERE) {
@@ -751,6 +760,11 @@ static int xfer(imexlb_handle *h, double *host, bool get)
     } else {
         e = hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) e = launch_set_f(c.dtype, P, h->stream);
+        if (e == hipSuccess && h->stage) {  // the x walls' copy of the new populations
+            FusedParams F;
+            fill_fused_params(h, F, h->cur, 1 - h->cur);
+            e = launch_faithful_stage_fill(F, h->stream);
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     hipFree(d);

@@ -321,7 +321,7 @@ struct CellCtx {
     uint8_t m;            // channel mask of the cell
     bool live;
     uint32_t rx0, ry0, rz0;  // BC_FAITHFUL: ring offsets of (y, z), (x, z), (x, y) on the x, y, z faces
-    uint32_t z;              // BC_FAITHFUL: the cell's z
+    uint32_t z, y;           // BC_FAITHFUL: the cell's z and y
 };
 
 template <int BC>
@@ -378,6 +378,7 @@ __device__ __forceinline__ void cell_setup(const FusedParams &P, uint32_t c, Cel
     k.z1 = z == nz - 1;
     if constexpr (BC == BC_FAITHFUL) {
         k.z = z;
+        k.y = y;
         k.rx0 = (z + 1) * (ny + 2) + y + 1;
         k.ry0 = (z + 1) * nx + x;
         k.rz0 = y * nx + x;
@@ -581,6 +582,25 @@ __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx
     const uint64_t ps = P.pstride;
     const uint32_t c = k.c;
     if constexpr (PAT == PAT_PULL) {
+        if constexpr (BC == BC_FAITHFUL) {
+            // the faithful cavity's x walls read these next step: a copy of
+            // x = 1 (e_x = -1) and x = nx - 2 (e_x = +1), each row's nine
+            // together
+            if (P.stage != nullptr) {
+                const uint64_t plane = (uint64_t)P.ny * P.nz, yz = (uint64_t)k.z * P.ny + k.y;
+                const uint32_t x = k.xv[1];
+                if (x == 1) {
+                    auto *p = glb(P.stage + yz * STAGE_ROW);
+#pragma unroll
+                    for (int r = 0; r < 9; ++r) p[r] = f[nth_dir<0, -1>(r)];
+                }
+                if (x + 2 == P.nx) {
+                    auto *p = glb(P.stage + (plane + yz) * STAGE_ROW);
+#pragma unroll
+                    for (int r = 0; r < 9; ++r) p[r] = f[nth_dir<0, 1>(r)];
+                }
+            }
+        }
 #pragma unroll
         for (int i = 0; i < Q; ++i) st_pop<S, true, AM, true>(dst, ps, i, c, f[i]);
     } else if constexpr (PAT == PAT_AA_EVEN) {

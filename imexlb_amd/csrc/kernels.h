@@ -71,6 +71,10 @@ inline int fused_addr_mode(uint64_t elem, uint64_t pstride, int am_max)
     return 0;
 }
 
+// Doubles per row of FusedParams::stage: the nine populations of one row of
+// the lattice, together.
+constexpr int STAGE_ROW = 9;
+
 // The cells per thread the fused kernels run with for a requested `cpt`:
 // two adjacent cells (3) only on periodic lattices with nx even and grouped
 // bases (address mode 2), two cells blockDim apart (2) only with grouped
@@ -115,6 +119,11 @@ struct FusedParams {
     int64_t ring_rel;        // (ring - src) in elements: ring loads address from the lattice's base
     uint32_t ring_n, ring_a, ring_b, ring_c;
     int faithful_sphere;     // BC_FAITHFUL: 0 the cavity, 1 the sphere channel
+    // faithful cavity: the populations the x walls read, written by each
+    // fused pass: stage[((side * nz + z) * ny + y) * STAGE_ROW + r] holds the
+    // r-th population with e_x = -1 at x = 1 (side 0) or with e_x = +1 at
+    // x = nx - 2 (side 1), each row's nine together; null elsewhere
+    double *stage;
     double u0;               // faithful sphere: inlet speed of the parabolic profile
     uint64_t seed;           // faithful sphere: the initial perturbation's seed
 };
@@ -139,6 +148,8 @@ __host__ __device__ __forceinline__ int64_t ring_index(const FusedParams &P, int
 hipError_t launch_faithful_boundaries(const FusedParams &P, hipStream_t st);
 // Initialize's values for the ring (0 in the cavity)
 hipError_t launch_faithful_ring_init(const FusedParams &P, hipStream_t st);
+// fills FusedParams::stage from the lattice P.src (after Initialize, set_f)
+hipError_t launch_faithful_stage_fill(const FusedParams &P, hipStream_t st);
 
 // The faithful sphere: solid iff (x-c)^2 + (y-c)^2 + (z-c)^2 <= r^2 with the
 // integer centre c = (glx - 1) / 8 on every axis and r = (glx - 1) / 64

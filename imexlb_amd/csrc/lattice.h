@@ -54,6 +54,14 @@ IMX_HD inline int nth_dir(int d)
     }
     return q;
 }
+// The rank of direction q among the nine with its e_x (e_x != 0): the d
+// with nth_dir<0, EX[q]>(d) == q.
+IMX_HD constexpr int x_rank(int q)
+{
+    int r = 0;
+    for (int k = 0; k < q; ++k) r += EX[k] == EX[q] ? 1 : 0;
+    return r;
+}
 IMX_HD inline void dir_of(int q, int (&e)[3], int &bq)
 {
     e[0] = e[1] = e[2] = bq = 0;
