@@ -643,8 +643,15 @@ __global__ void k_faithful_stage_fill(const FusedParams P)
     const uint64_t row = t / STAGE_ROW;  // side * plane + (z ny + y)
     const int side = row >= plane ? 1 : 0;
     const uint64_t yz = row - (side ? plane : 0);
-    const int q = side == 0 ? nth_dir<0, -1>(r) : nth_dir<0, 1>(r);
-    const uint32_t x = side == 0 ? 1u : (uint32_t)P.nx - 2u;
+    int q;
+    uint32_t x;
+    if (P.faithful_sphere) {  // inlet x = 0, e_x > 0; outlet x = nx - 1, e_x < 0
+        q = side == 0 ? nth_dir<0, 1>(r) : nth_dir<0, -1>(r);
+        x = side == 0 ? 0u : (uint32_t)P.nx - 1u;
+    } else {  // walls: x = 1, e_x < 0; x = nx - 2, e_x > 0
+        q = side == 0 ? nth_dir<0, -1>(r) : nth_dir<0, 1>(r);
+        x = side == 0 ? 1u : (uint32_t)P.nx - 2u;
+    }
     P.stage[t] = ((const double *)P.src)[(uint64_t)q * P.pstride + yz * P.nx + x];
 }
 
@@ -684,9 +691,17 @@ __global__ void k_faithful_inlet_outlet(const FusedParams P)
     int e[3], bq;
     dir_of(q, e, bq);
     const int j = t % na - 1, k = t / na - 1;
-    const GhostSrc g = OUTLET ? faithful_src(P, q, P.nx - 1, j + e[1], k + e[2]) : faithful_src(P, q, 0, j, k);
-    double v = *g.p;
-    v = g.is_deep ? g.deep : v;
+    const int sj = OUTLET ? j + e[1] : j, sk = OUTLET ? k + e[2] : k;
+    double v;
+    if (P.stage != nullptr && sj >= 0 && sj < P.ny && sk >= 0 && sk < P.nz) {
+        // an interior cell of plane 0 or nx - 1: from the copy the last fused
+        // pass wrote
+        v = P.stage[(((uint64_t)OUTLET * P.nz + sk) * P.ny + sj) * STAGE_ROW + x_rank(q)];
+    } else {
+        const GhostSrc g = faithful_src(P, q, OUTLET ? P.nx - 1 : 0, sj, sk);
+        v = *g.p;
+        v = g.is_deep ? g.deep : v;
+    }
     const uint64_t r = (uint64_t)ring_index(P, OUTLET ? P.nx : -1, j, k);
     P.ring[(uint64_t)q * P.ring_n + r] = v;
 }

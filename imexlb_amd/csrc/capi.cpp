@@ -45,7 +45,7 @@ struct imexlb_handle {
     double feq_in[Q] = {};
     double *tgv_tab = nullptr;
     double *ring = nullptr;  // faithful cavity: the first ghost layer (FusedParams::ring)
-    double *stage = nullptr; // faithful cavity: what the x walls read (FusedParams::stage)
+    double *stage = nullptr; // faithful mode: what the x-face boundary kernels read (FusedParams::stage)
     uint32_t ring_a = 0, ring_b = 0, ring_c = 0;
     int block = 256, cpt = 1, xcd = 1, ntl = 0;
     int64_t steps = 0;
@@ -412,11 +412,9 @@ int imexlb_create(const imexlb_config *cfg, imexlb_handle **out)
         const size_t rb = (size_t)Q * 2 * (h->ring_a + h->ring_b + h->ring_c) * 8;
         if (hipMalloc(&h->ring, rb) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "ghost ring"));
         h->device_bytes += (int64_t)rb;
-        if (c.case_id == IMEXLB_CASE_CAVITY) {
-            const size_t sb = (size_t)2 * STAGE_ROW * c.ny * c.nz * 8;
-            if (hipMalloc(&h->stage, sb) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "x-wall stage"));
-            h->device_bytes += (int64_t)sb;
-        }
+        const size_t sb = (size_t)2 * STAGE_ROW * c.ny * c.nz * 8;
+        if (hipMalloc(&h->stage, sb) != hipSuccess) return bail(fail(h, IMEXLB_ENOMEM, "x-face stage"));
+        h->device_bytes += (int64_t)sb;
     }
 
     // boundary constants in the oracle's operation order
@@ -760,7 +758,7 @@ static int xfer(imexlb_handle *h, double *host, bool get)
     } else {
         e = hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, h->stream);
         if (e == hipSuccess) e = launch_set_f(c.dtype, P, h->stream);
-        if (e == hipSuccess && h->stage) {  // the x walls' copy of the new populations
+        if (e == hipSuccess && h->stage) {  // the x-face kernels' copy of the new populations
             FusedParams F;
             fill_fused_params(h, F, h->cur, 1 - h->cur);
             e = launch_faithful_stage_fill(F, h->stream);

@@ -583,21 +583,33 @@ __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx
     const uint32_t c = k.c;
     if constexpr (PAT == PAT_PULL) {
         if constexpr (BC == BC_FAITHFUL) {
-            // the faithful cavity's x walls read these next step: a copy of
-            // x = 1 (e_x = -1) and x = nx - 2 (e_x = +1), each row's nine
-            // together
+            // what the next step's x-face boundary kernels read, each row's
+            // nine together (FusedParams::stage): the cavity's walls x = 1
+            // (e_x = -1) and x = nx - 2 (e_x = +1), the sphere's inlet x = 0
+            // (e_x = +1) and outlet x = nx - 1 (e_x = -1)
             if (P.stage != nullptr) {
                 const uint64_t plane = (uint64_t)P.ny * P.nz, yz = (uint64_t)k.z * P.ny + k.y;
                 const uint32_t x = k.xv[1];
-                if (x == 1) {
-                    auto *p = glb(P.stage + yz * STAGE_ROW);
+                auto *p0 = glb(P.stage + yz * STAGE_ROW);
+                auto *p1 = glb(P.stage + (plane + yz) * STAGE_ROW);
+                if (P.faithful_sphere) {
+                    if (x == 0) {
 #pragma unroll
-                    for (int r = 0; r < 9; ++r) p[r] = f[nth_dir<0, -1>(r)];
-                }
-                if (x + 2 == P.nx) {
-                    auto *p = glb(P.stage + (plane + yz) * STAGE_ROW);
+                        for (int r = 0; r < 9; ++r) p0[r] = f[nth_dir<0, 1>(r)];
+                    }
+                    if (x + 1 == P.nx) {
 #pragma unroll
-                    for (int r = 0; r < 9; ++r) p[r] = f[nth_dir<0, 1>(r)];
+                        for (int r = 0; r < 9; ++r) p1[r] = f[nth_dir<0, -1>(r)];
+                    }
+                } else {
+                    if (x == 1) {
+#pragma unroll
+                        for (int r = 0; r < 9; ++r) p0[r] = f[nth_dir<0, -1>(r)];
+                    }
+                    if (x + 2 == P.nx) {
+#pragma unroll
+                        for (int r = 0; r < 9; ++r) p1[r] = f[nth_dir<0, 1>(r)];
+                    }
                 }
             }
         }

@@ -119,10 +119,13 @@ struct FusedParams {
     int64_t ring_rel;        // (ring - src) in elements: ring loads address from the lattice's base
     uint32_t ring_n, ring_a, ring_b, ring_c;
     int faithful_sphere;     // BC_FAITHFUL: 0 the cavity, 1 the sphere channel
-    // faithful cavity: the populations the x walls read, written by each
-    // fused pass: stage[((side * nz + z) * ny + y) * STAGE_ROW + r] holds the
-    // r-th population with e_x = -1 at x = 1 (side 0) or with e_x = +1 at
-    // x = nx - 2 (side 1), each row's nine together; null elsewhere
+    // faithful mode: the populations the x-face boundary kernels read,
+    // written by each fused pass: stage[((side * nz + z) * ny + y) * STAGE_ROW
+    // + r] holds, for the cavity's walls, the r-th population with e_x = -1
+    // at x = 1 (side 0) or with e_x = +1 at x = nx - 2 (side 1); for the
+    // sphere's inlet and outlet copies, the r-th with e_x = +1 at x = 0 (side
+    // 0) or with e_x = -1 at x = nx - 1 (side 1); each row's nine together;
+    // null elsewhere
     double *stage;
     double u0;               // faithful sphere: inlet speed of the parabolic profile
     uint64_t seed;           // faithful sphere: the initial perturbation's seed
