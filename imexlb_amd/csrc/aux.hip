@@ -557,8 +557,12 @@ hipError_t launch_faithful_ring_init(const FusedParams &P, hipStream_t st)
 // The one-rank unpack of lid_driven_cavity/src/mpi_view_transfer.cpp:617-688
 // writes receive buffers nothing filled (zeros) into the ghost planes
 // y = ny, z = nz, x = -1 and x = nx, all 27 populations: here their cells of
-// the first layer.  Thread t: the two x faces, then the plane y = ny (x in
-// [0, nx)), then the plane z = nz (x in [0, nx), y in [-1, ny]).
+// the first layer.  Of a cell's 27 entries only those a wall writes later in
+// the step (the directions pointing into the cavity across any of the cell's
+// ghost axes) ever hold anything but zero: Initialize zeroes the ring and
+// nothing else writes it.  Those are zeroed again here; the others already
+// are.  Thread t: the two x faces, then the plane y = ny (x in [0, nx)),
+// then the plane z = nz (x in [0, nx), y in [-1, ny]).
 __global__ void k_faithful_unpack(const FusedParams P)
 {
     const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,8 +587,16 @@ __global__ void k_faithful_unpack(const FusedParams P)
         return;
     }
     const uint64_t r = (uint64_t)ring_index(P, i, j, k);
+    // the inward direction across each ghost axis of the cell (0: not a
+    // ghost coordinate)
+    const int sx = i < 0 ? 1 : (i >= P.nx ? -1 : 0);
+    const int sy = j < 0 ? 1 : (j >= P.ny ? -1 : 0);
+    const int sz = k < 0 ? 1 : (k >= P.nz ? -1 : 0);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) P.ring[(uint64_t)q * P.ring_n + r] = 0.0;
+    for (int q = 0; q < Q; ++q) {
+        const bool walled = (sx != 0 && EX[q] == sx) || (sy != 0 && EY[q] == sy) || (sz != 0 && EZ[q] == sz);
+        if (walled) P.ring[(uint64_t)q * P.ring_n + r] = 0.0;
+    }
 }
 
 // One wall of lid_driven_cavity/src/lbm.cpp:233-301: for the first-layer
