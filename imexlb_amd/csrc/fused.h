@@ -587,29 +587,23 @@ __device__ __forceinline__ void cell_scatter(const FusedParams &P, const CellCtx
             // nine together (FusedParams::stage): the cavity's walls x = 1
             // (e_x = -1) and x = nx - 2 (e_x = +1), the sphere's inlet x = 0
             // (e_x = +1) and outlet x = nx - 1 (e_x = -1)
+            // The register array f is indexed with compile-time constants
+            // only (XDIR_P / XDIR_M), the two cases chosen by selects: an index
+            // computed at run time would move f to private memory (64 more
+            // bytes written per cell).
             if (P.stage != nullptr) {
                 const uint64_t plane = (uint64_t)P.ny * P.nz, yz = (uint64_t)k.z * P.ny + k.y;
                 const uint32_t x = k.xv[1];
-                auto *p0 = glb(P.stage + yz * STAGE_ROW);
-                auto *p1 = glb(P.stage + (plane + yz) * STAGE_ROW);
-                if (P.faithful_sphere) {
-                    if (x == 0) {
+                const bool sph = P.faithful_sphere != 0;
+                if (x == (sph ? 0u : 1u)) {
+                    auto *p0 = glb(P.stage + yz * STAGE_ROW);
 #pragma unroll
-                        for (int r = 0; r < 9; ++r) p0[r] = f[nth_dir<0, 1>(r)];
-                    }
-                    if (x + 1 == P.nx) {
+                    for (int r = 0; r < 9; ++r) p0[r] = sph ? f[XDIR_P[r]] : f[XDIR_M[r]];
+                }
+                if (x == (sph ? P.nx - 1u : P.nx - 2u)) {
+                    auto *p1 = glb(P.stage + (plane + yz) * STAGE_ROW);
 #pragma unroll
-                        for (int r = 0; r < 9; ++r) p1[r] = f[nth_dir<0, -1>(r)];
-                    }
-                } else {
-                    if (x == 1) {
-#pragma unroll
-                        for (int r = 0; r < 9; ++r) p0[r] = f[nth_dir<0, -1>(r)];
-                    }
-                    if (x + 2 == P.nx) {
-#pragma unroll
-                        for (int r = 0; r < 9; ++r) p1[r] = f[nth_dir<0, 1>(r)];
-                    }
+                    for (int r = 0; r < 9; ++r) p1[r] = sph ? f[XDIR_M[r]] : f[XDIR_P[r]];
                 }
             }
         }

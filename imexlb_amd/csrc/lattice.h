@@ -54,6 +54,31 @@ IMX_HD inline int nth_dir(int d)
     }
     return q;
 }
+// The nine directions with e_x = +1 and with e_x = -1, in index order
+// (nth_dir<0, 1> and nth_dir<0, -1>), as tables: kernels that index a
+// register array with them need the index known at compile time.
+constexpr int XDIR_P[9] = {1, 7, 9, 11, 13, 19, 21, 23, 25};
+constexpr int XDIR_M[9] = {2, 8, 10, 12, 14, 20, 22, 24, 26};
+constexpr bool xdir_tables_ok()
+{
+    for (int r = 0; r < 9; ++r) {
+        int p = 0, m = 0, cp = 0, cm = 0;
+        for (int k = 0; k < Q; ++k) {
+            if (EX[k] > 0) {
+                if (cp == r) p = k;
+                ++cp;
+            }
+            if (EX[k] < 0) {
+                if (cm == r) m = k;
+                ++cm;
+            }
+        }
+        if (XDIR_P[r] != p || XDIR_M[r] != m) return false;
+    }
+    return true;
+}
+static_assert(xdir_tables_ok(), "XDIR_P / XDIR_M must list the e_x = +1 / -1 directions in index order");
+
 // The rank of direction q among the nine with its e_x (e_x != 0): the d
 // with nth_dir<0, EX[q]>(d) == q.
 IMX_HD constexpr int x_rank(int q)
