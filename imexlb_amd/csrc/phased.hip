@@ -135,7 +135,10 @@ __global__ void k_init(const Dims d, const Arrays a)
     }
 }
 
-__global__ void k_collide(const Dims d, const Arrays a)
+// 27 values and 27 addresses live at once: the bound of 256 threads (every
+// launch here) leaves room for them; the default bound caps the registers at
+// 128 and spills three values to private memory
+__global__ void __launch_bounds__(256) k_collide(const Dims d, const Arrays a)
 {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.ncell) return;
