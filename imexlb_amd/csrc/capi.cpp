@@ -163,6 +163,8 @@ static int validate(const imexlb_config *c)
     if (c->cells_per_thread < 0 || c->cells_per_thread > 3)
         return fail(nullptr, IMEXLB_EINVAL, "cells_per_thread %d", c->cells_per_thread);
     if (c->addr_mode < -1 || c->addr_mode > 2) return fail(nullptr, IMEXLB_EINVAL, "addr_mode %d", c->addr_mode);
+    if (c->xcd_remap < -1 || c->xcd_remap > 1) return fail(nullptr, IMEXLB_EINVAL, "xcd_remap %d", c->xcd_remap);
+    if (c->nt_loads < -1 || c->nt_loads > 1) return fail(nullptr, IMEXLB_EINVAL, "nt_loads %d", c->nt_loads);
     if (c->engine == IMEXLB_ENGINE_FUSED) {
         if (c->mode == IMEXLB_MODE_FAITHFUL && c->case_id != IMEXLB_CASE_TGV &&
             (c->dtype != IMEXLB_F64 || c->storage != IMEXLB_PULL))

@@ -33,6 +33,12 @@ int main(int argc, char **argv)
     c.cells_per_thread = 4; /* 1, 2 or 3 cells per thread (0: default) */
     CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
     c.cells_per_thread = 0;
+    c.xcd_remap = 2; /* -1 default, 0 or 1 */
+    CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
+    c.xcd_remap = -1;
+    c.nt_loads = 5; /* -1 default, 0 or 1 */
+    CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
+    c.nt_loads = -1;
     c.nx = 1; /* extents below 2 are refused before any device call */
     CHECK(imexlb_create(&c, &h) == IMEXLB_EINVAL && h == NULL);
     CHECK(strlen(imexlb_last_error(NULL)) > 0);
