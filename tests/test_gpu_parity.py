@@ -492,6 +492,25 @@ def test_fused_faithful_sphere_bitexact(n):
     assert same(o.get_f(), g.get_f())
 
 
+@pytest.mark.parametrize("n", [(40, 14, 12), (72, 9, 12)])
+@pytest.mark.parametrize("shape", [(64, 2, 0, 0), (128, 2, 1, 1), (512, 1, 0, 1)])
+def test_fused_faithful_sphere_launch_shapes_bitexact(n, shape):
+    """The faithful sphere's fused pass with one and two cells per thread,
+    linear and XCD tile orders: the pass also writes the staged x = 0 and
+    x = nx - 1 populations that the inlet and outlet copies read, so every
+    shape must give the oracle's bits (the second extent puts the ball into
+    the first ghost layer)."""
+    o, g = make_pair(SPHERE, FAITHFUL, n, 0.08, 0.1, engine="fused")
+    g.set_launch(*shape)
+    assert g.info()["cells_per_thread"] == shape[1]
+    for k in (1, 6, 17):
+        o.run(k)
+        g.step(k)
+        moments_equal(o, g)
+    o.Collision()
+    assert same(o.get_f(), g.get_f())
+
+
 def test_fused_faithful_cavity_six_calls_and_launch_shapes(tmp_path):
     """The six calls per step, and every launch shape, give the same bits;
     restart files are refused (they would need the ghost ring)."""
