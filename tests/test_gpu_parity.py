@@ -538,6 +538,9 @@ def test_fused_faithful_cavity_six_calls_and_launch_shapes(tmp_path):
             ref = m
         for a, b in zip(ref, m):
             assert same(a, b), shape
+    o.run(8)  # 13 steps, as every shape ran
+    for a, b in zip(o.moments(), ref):
+        assert same(a, b)
 
 
 @pytest.mark.parametrize("case,n", [(LDC, (12, 10, 14)), (SPHERE, (40, 14, 12))])
