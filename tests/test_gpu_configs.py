@@ -89,7 +89,20 @@ def test_cavity128_faithful_fused_first_steps_bitexact():
     g.step(40)
     fields_equal(o, g, "cavity128 faithful fused")
     o.Collision()
-    assert np.array_equal(o.get_f(), g.get_f())
+    fg = g.get_f()
+    assert np.array_equal(o.get_f(), fg)
+    # two cells per thread, both tile orders: the same bits as the shape above
+    mg = g.moments()
+    for shape in [(128, 2, 1, 1), (64, 2, 0, 0)]:
+        h = LBM(n, n, n, 0.096, u0=0.1, case="cavity", mode="faithful", engine="fused")
+        h.set_launch(*shape)
+        assert h.info()["cells_per_thread"] == 2
+        h.Initialize()
+        h.step(40)
+        for a, b in zip(mg, h.moments()):
+            assert np.array_equal(a, b), shape
+        assert np.array_equal(fg, h.get_f()), shape
+        h.close()
 
 
 def test_reference_sphere_input_faithful_both_engines():
